@@ -1,0 +1,226 @@
+/*
+ * libscrna -- MI355X (gfx950) Chemical Langevin Equation simulator for
+ * synthetic scRNA-seq, C ABI.
+ *
+ * This is the drop-in boundary for the GPU CLE path of SyntheticscRNAseq.jl.
+ * A Julia shim `ccall`s these entry points in place of the CUDA.jl package
+ * extension ext/SyntheticscRNAseqCUDAExt.jl; every entry point below names
+ * the reference method it replaces.
+ *
+ * Conventions
+ *  - Plain C types only.  Every pointer is a host pointer unless the name
+ *    says `_dev`.  Doubles in, doubles or int32 out, like the reference.
+ *  - Gene and cell indices are 0-based (the Julia shim subtracts 1).
+ *  - Matrices use Julia's column-major layout.  The interaction matrix is
+ *    A[i + j*G] = effect of protein j on gene i (network.jl:43-45).  Result
+ *    matrices are cells x G column-major, i.e. element (cell c, gene g) at
+ *    out[g*ld + c], which is exactly `Matrix{Float64}(cell_num, G)` with
+ *    ld = cell_num.
+ *  - Every function returns SCRNA_OK (0) or a negative scrna_status;
+ *    scrna_last_error() returns a thread-local message for the last failure.
+ *  - Blocking calls never keep host pointers after they return.
+ */
+#ifndef SCRNA_H
+#define SCRNA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SCRNA_ABI_VERSION 1
+
+typedef enum scrna_status {
+    SCRNA_OK = 0,
+    SCRNA_ERR_INVALID = -1,     /* invalid argument (Julia: ArgumentError / AssertionError) */
+    SCRNA_ERR_BUDGET = -2,      /* request exceeds the context's HBM budget */
+    SCRNA_ERR_DEVICE = -3,      /* HIP runtime / device failure */
+    SCRNA_ERR_CANCELLED = -4,   /* serving request was cancelled */
+    SCRNA_ERR_TIMEOUT = -5,     /* scrna_wait timed out */
+    SCRNA_ERR_UNSUPPORTED = -6  /* combination not supported (Julia: error()) */
+} scrna_status;
+
+typedef enum scrna_readout {
+    SCRNA_READOUT_MRNA = 0,     /* :mrna */
+    SCRNA_READOUT_PROTEIN = 1,  /* :protein (reference default, cle.jl:37 / ext:52) */
+    SCRNA_READOUT_BOTH = 2      /* :both -> cells x 2G, mRNA columns first (cle.jl:128) */
+} scrna_readout;
+
+typedef enum scrna_precision {
+    SCRNA_PREC_FP32 = 0,        /* FP32 state, regulation on the VALU */
+    SCRNA_PREC_BF16 = 1,        /* dense regulation on BF16 MFMA, FP32 accumulate */
+    SCRNA_PREC_BF16X3 = 2       /* dense regulation as hi*hi + hi*lo + lo*hi BF16 MFMA */
+} scrna_precision;
+
+/* GeneNetwork (network.jl:55-63) with CoopEdge / RedunEdge (network.jl:16-34).
+ * Gate sources are listed in CSR form: gate e has sources
+ * src[off[e] .. off[e+1]-1].  k_on / k_off may be NULL (all constitutive);
+ * INFINITY marks a constitutive gene (network.jl:48-49). */
+typedef struct scrna_network {
+    int32_t G;
+    const double* basals;         /* [G] */
+    const double* A;              /* [G*G] column-major, diagonal must be 0 */
+    int32_t n_coop;
+    const int32_t* coop_target;   /* [n_coop] */
+    const int32_t* coop_off;      /* [n_coop+1] */
+    const int32_t* coop_src;
+    const double* coop_strength;  /* [n_coop] */
+    int32_t n_redun;
+    const int32_t* redun_target;
+    const int32_t* redun_off;
+    const int32_t* redun_src;
+    const double* redun_strength;
+    const double* k_on;           /* [G] or NULL */
+    const double* k_off;          /* [G] or NULL */
+} scrna_network;
+
+/* KineticParams (kinetics.jl:25-32). */
+typedef struct scrna_kinetics {
+    double k_t, K_d, n, mu_m, mu_p, dilution;
+} scrna_kinetics;
+
+/* PopulationConfig (population.jl:27-45) plus the two choices the three
+ * reference implementations disagree on. */
+typedef struct scrna_population {
+    int32_t enabled;
+    double growth_rate;
+    double V_div;
+    double V_init_lo, V_init_hi;
+    int32_t div_check_interval;
+    int32_t partition;            /* 0: halve (cle.jl:249-261, ext:494-505); 1: Binomial(round(x), 1/2) (population.jl:423-440) */
+    int32_t replace_mode;         /* 0: uniform over 1:N (population.jl:448, ext:509); 1: exclude mother (cle.jl:265-268) */
+} scrna_population;
+
+/* CaptureModel (population.jl:57-74). */
+typedef struct scrna_capture {
+    int32_t enabled;
+    double efficiency;
+    double efficiency_std;
+    int32_t readout;              /* SCRNA_READOUT_MRNA or SCRNA_READOUT_PROTEIN */
+} scrna_capture;
+
+/* Run options: CLE(dt) (cle.jl:21-23), cell_num, T, readout, plus the
+ * options the reference GPU path lacks (seed, init, snapshots, precision). */
+typedef struct scrna_run {
+    double dt;
+    double T;                     /* n_steps = ceil(T/dt) in double (cle.jl:56, ext:69) */
+    int64_t cell_num;
+    int32_t readout;              /* scrna_readout */
+    int32_t init_mode;            /* 0: ceil(steady state) (cle.jl:52-53); 1: zeros */
+    uint64_t seed;                /* Philox key */
+    uint32_t net_id;              /* stream id of this network inside its request */
+    int32_t precision;            /* scrna_precision */
+    int64_t snapshot_every;       /* steps between snapshots; 0 = none */
+    const float* injected_noise;  /* optional, host: [((s*G+g)*2+k)*cell_num + c], k=0 mRNA, 1 protein */
+} scrna_run;
+
+/* Outputs of one network (all optional except y).
+ *  y:          readout, cells x G (x2G for BOTH) column-major, leading dim ld
+ *  counts:     captured counts, cells x G column-major int32, leading dim ld
+ *  snapshots:  [n_snap][2][G][cell_num] float (species 0 = mRNA, 1 = protein)
+ *  moments:    [n_snap_or_1][G][5] double sums over cells: m, m^2, p, p^2, m*p
+ *              (final state when there are no snapshots)
+ *  volumes:    [cell_num] final cell volumes (population runs) */
+typedef struct scrna_output {
+    double* y;
+    int64_t ld;
+    int32_t* counts;
+    float* snapshots;
+    double* moments;
+    double* volumes;
+} scrna_output;
+
+typedef struct scrna_ctx scrna_ctx;
+typedef struct scrna_plan scrna_plan;
+
+/* ---- library -------------------------------------------------------- */
+int32_t scrna_abi_version(void);
+const char* scrna_version(void);
+const char* scrna_last_error(void);
+
+/* Context = one device + one HBM budget.  hbm_budget_bytes 0 = 90% of free. */
+int scrna_ctx_create(int32_t device, uint64_t hbm_budget_bytes, scrna_ctx** out);
+int scrna_ctx_destroy(scrna_ctx* ctx);
+
+/* ---- blocking drop-ins ---------------------------------------------- */
+
+/* Replaces SyntheticscRNAseq.simulate(net, ::CLE, kin, ::Val{:gpu};
+ * cell_num, T, readout)  (ext/SyntheticscRNAseqCUDAExt.jl:48-117), and with
+ * pop != NULL the population method (ext:409-529).  capture != NULL fuses
+ * apply_capture (capture.jl:25-47) on the final state. */
+int scrna_simulate_cle(scrna_ctx* ctx, const scrna_network* net, const scrna_kinetics* kin,
+                       const scrna_run* run, const scrna_population* pop,
+                       const scrna_capture* capture, scrna_output* out);
+
+/* Replaces SyntheticscRNAseq.simulate_gpu_batch(networks, ::CLE, kin;
+ * cell_num, T, readout)  (ext:132-213).  Unlike the reference, networks may
+ * differ in G, cell_num, kinetics and horizon: runs[i] belongs to nets[i]. */
+int scrna_simulate_cle_batch(scrna_ctx* ctx, int32_t n, const scrna_network* nets,
+                             const scrna_kinetics* kins, const scrna_run* runs,
+                             const scrna_population* pops, const scrna_capture* captures,
+                             scrna_output* outs);
+
+/* Replaces apply_capture(X, CaptureModel; rng)  (capture.jl:25-47) on the
+ * GPU.  X is cells x G column-major double (ld >= cells); Y int32 same
+ * layout.  The random stream is keyed by (seed, net_id). */
+int scrna_apply_capture(scrna_ctx* ctx, const double* X, int64_t cells, int32_t G, int64_t ld,
+                        double efficiency, double efficiency_std, uint64_t seed, uint32_t net_id,
+                        int32_t* Y);
+
+/* ---- device-resident plans (benchmarks, graphs, serving) ------------ */
+
+/* Compile and upload n networks, allocate their device outputs. */
+int scrna_plan_create(scrna_ctx* ctx, int32_t n, const scrna_network* nets,
+                      const scrna_kinetics* kins, const scrna_run* runs,
+                      const scrna_population* pops, const scrna_capture* captures,
+                      scrna_plan** out);
+/* Enqueue the whole simulation on `stream` (a hipStream_t; NULL = the
+ * context's stream).  Allocates nothing; may be captured into a hipGraph. */
+int scrna_plan_launch(scrna_plan* plan, void* stream);
+/* Wait for the plan's stream. */
+int scrna_plan_sync(scrna_plan* plan);
+/* Copy network i's results to host outputs. */
+int scrna_plan_download(scrna_plan* plan, int32_t i, scrna_output* out);
+/* Milliseconds of the last launch's stepper kernels (HIP events recorded
+ * on the launch stream around them), and the cell*gene*steps it covered. */
+int scrna_plan_stepper_ms(scrna_plan* plan, double* ms, double* cell_gene_steps);
+/* Device bytes held by the plan. */
+int scrna_plan_bytes(scrna_plan* plan, uint64_t* bytes);
+int scrna_plan_destroy(scrna_plan* plan);
+
+/* ---- serving: continuous batching of simulate() requests ------------ */
+
+typedef struct scrna_server scrna_server;
+
+typedef struct scrna_request {
+    scrna_network net;
+    scrna_kinetics kin;
+    scrna_run run;
+    scrna_capture capture;
+} scrna_request;
+
+typedef struct scrna_latency {
+    double p50_ms, p99_ms, mean_ms, max_ms;
+    double cell_gene_steps;        /* completed work */
+    double busy_s;                 /* wall time with work on the device */
+    int64_t completed, rejected, cancelled;
+} scrna_latency;
+
+/* Start the device-resident request loop.  max_slots bounds concurrent
+ * requests; admission is by the context's HBM budget. */
+int scrna_server_create(scrna_ctx* ctx, int32_t max_slots, scrna_server** out);
+/* Copies the request (network arrays included); returns a ticket id. */
+int scrna_submit(scrna_server* srv, const scrna_request* req, int64_t* ticket);
+/* Block until the ticket finishes (timeout_ms < 0: forever), then copy its
+ * results to out.  Returns SCRNA_ERR_TIMEOUT if still running. */
+int scrna_wait(scrna_server* srv, int64_t ticket, int64_t timeout_ms, scrna_output* out);
+int scrna_cancel(scrna_server* srv, int64_t ticket);
+int scrna_server_stats(scrna_server* srv, scrna_latency* out);
+int scrna_server_destroy(scrna_server* srv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

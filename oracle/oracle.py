@@ -1,0 +1,227 @@
+"""ctypes wrapper of the FP64 CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg may import this module.  It is the checker, never the thing
+measured or shipped.  See scrna_oracle.h for the reference files restated.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_fp = C.POINTER(C.c_float)
+_up = C.POINTER(C.c_uint8)
+
+
+class OrcNetwork(C.Structure):
+    _fields_ = [("G", C.c_int32), ("basals", _dp), ("A", _dp),
+                ("n_coop", C.c_int32), ("coop_target", _ip), ("coop_off", _ip), ("coop_src", _ip),
+                ("coop_strength", _dp),
+                ("n_redun", C.c_int32), ("redun_target", _ip), ("redun_off", _ip), ("redun_src", _ip),
+                ("redun_strength", _dp),
+                ("k_on", _dp), ("k_off", _dp)]
+
+
+class OrcKinetics(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("k_t", "K_d", "n", "mu_m", "mu_p", "dilution")]
+
+
+class OrcPopulation(C.Structure):
+    _fields_ = [("enabled", C.c_int32), ("growth_rate", C.c_double), ("V_div", C.c_double),
+                ("V_init_lo", C.c_double), ("V_init_hi", C.c_double), ("div_check_interval", C.c_int32),
+                ("partition", C.c_int32), ("replace_mode", C.c_int32)]
+
+
+class OrcRun(C.Structure):
+    _fields_ = [("dt", C.c_double), ("n_steps", C.c_int64), ("cell_num", C.c_int64),
+                ("cell_lo", C.c_int64), ("cell_hi", C.c_int64), ("seed", C.c_uint64),
+                ("net_id", C.c_uint32), ("init_mode", C.c_int32), ("snapshot_every", C.c_int64),
+                ("injected", _fp), ("n_threads", C.c_int32)]
+
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        lib = C.CDLL(LIB)
+        lib.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        lib.orc_normal_pair.argtypes = [C.c_uint32, C.c_uint32, _dp, _dp]
+        lib.orc_regulation_matrix.argtypes = [C.POINTER(OrcNetwork), C.POINTER(OrcKinetics), _dp, _dp]
+        lib.orc_regulation_loop.argtypes = [C.POINTER(OrcNetwork), C.POINTER(OrcKinetics), _dp, _dp]
+        lib.orc_cle_simulate.argtypes = [C.POINTER(OrcNetwork), C.POINTER(OrcKinetics), C.POINTER(OrcRun),
+                                         _dp, _dp, _dp, _dp, _up]
+        lib.orc_cle_simulate.restype = C.c_int
+        lib.orc_cle_population.argtypes = [C.POINTER(OrcNetwork), C.POINTER(OrcKinetics), C.POINTER(OrcRun),
+                                           C.POINTER(OrcPopulation), _dp, _dp, _dp, _up]
+        lib.orc_cle_population.restype = C.c_int
+        lib.orc_binomial.argtypes = [C.c_int64, C.c_double, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64]
+        lib.orc_binomial.restype = C.c_int64
+        lib.orc_capture.argtypes = [_dp, C.c_int64, C.c_int32, C.c_int64, C.c_double, C.c_double, C.c_uint64,
+                                    C.c_uint32, _ip, _dp]
+        lib.orc_capture.restype = C.c_int
+        lib.orc_moments.argtypes = [_dp, _dp, C.c_int64, C.c_int32, C.c_int64, _dp]
+        _lib = lib
+    return _lib
+
+
+def _net(net):
+    """Build an OrcNetwork from any object with the GeneNetwork attributes."""
+    G = int(net.G)
+    basals = np.ascontiguousarray(net.basals, dtype=np.float64)
+    A = np.asfortranarray(np.asarray(net.interactions, dtype=np.float64)).ravel(order="F")
+    keep = [basals, A]
+
+    def gates(edges):
+        edges = list(edges)
+        if not edges:
+            return 0, None, None, None, None
+        tgt = np.array([e.target for e in edges], dtype=np.int32)
+        off = np.zeros(len(edges) + 1, dtype=np.int32)
+        off[1:] = np.cumsum([len(e.sources) for e in edges])
+        src = np.array([s for e in edges for s in e.sources], dtype=np.int32)
+        st = np.array([e.strength for e in edges], dtype=np.float64)
+        keep.extend([tgt, off, src, st])
+        return (len(edges), tgt.ctypes.data_as(_ip), off.ctypes.data_as(_ip), src.ctypes.data_as(_ip),
+                st.ctypes.data_as(_dp))
+
+    nc = gates(getattr(net, "cooperative", []))
+    nr = gates(getattr(net, "redundant", []))
+    kon = getattr(net, "k_on", None)
+    koff = getattr(net, "k_off", None)
+    konp = koffp = None
+    if kon is not None:
+        kon = np.ascontiguousarray(kon, dtype=np.float64)
+        koff = np.ascontiguousarray(koff, dtype=np.float64)
+        keep.extend([kon, koff])
+        konp, koffp = kon.ctypes.data_as(_dp), koff.ctypes.data_as(_dp)
+    s = OrcNetwork(G, basals.ctypes.data_as(_dp), A.ctypes.data_as(_dp), *nc, *nr, konp, koffp)
+    return s, keep
+
+
+def _kin(kin):
+    return OrcKinetics(kin.k_t, kin.K_d, kin.n, kin.mu_m, kin.mu_p, kin.dilution)
+
+
+def philox(ctr, key):
+    lib = load()
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib.orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def normal_pair(a, b):
+    lib = load()
+    z0, z1 = C.c_double(), C.c_double()
+    lib.orc_normal_pair(a, b, C.byref(z0), C.byref(z1))
+    return z0.value, z1.value
+
+
+def regulation(net, kin, p, form="matrix"):
+    lib = load()
+    s, keep = _net(net)
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    out = np.empty(net.G)
+    fn = lib.orc_regulation_matrix if form == "matrix" else lib.orc_regulation_loop
+    fn(C.byref(s), C.byref(_kin(kin)), p.ctypes.data_as(_dp), out.ctypes.data_as(_dp))
+    return out
+
+
+def cle(net, kin, *, dt, T=None, n_steps=None, cell_num, cell_lo=0, cell_hi=None, seed=0, net_id=0,
+        init="steady", snapshot_every=0, injected=None, threads=1):
+    """FP64 CLE of cells [cell_lo, cell_hi).  Returns dict(m, p, snaps_m, snaps_p, promoter)
+    with arrays shaped (G, n_cells) / (n_snap, G, n_cells)."""
+    lib = load()
+    if n_steps is None:
+        n_steps = int(np.ceil(T / dt))
+    cell_hi = cell_num if cell_hi is None else cell_hi
+    nc = cell_hi - cell_lo
+    G = net.G
+    s, keep = _net(net)
+    inj_p = None
+    if injected is not None:
+        inj = np.ascontiguousarray(injected, dtype=np.float32)
+        keep.append(inj)
+        inj_p = inj.ctypes.data_as(_fp)
+    run = OrcRun(dt, n_steps, cell_num, cell_lo, cell_hi, seed & 0xFFFFFFFFFFFFFFFF, net_id,
+                 0 if init == "steady" else 1, snapshot_every, inj_p, threads)
+    n_snap = n_steps // snapshot_every if snapshot_every > 0 else 0
+    m = np.empty((G, nc))
+    p = np.empty((G, nc))
+    sm = np.empty((max(n_snap, 1), G, nc))
+    sp = np.empty((max(n_snap, 1), G, nc))
+    pr = np.empty((G, nc), dtype=np.uint8)
+    rc = lib.orc_cle_simulate(C.byref(s), C.byref(_kin(kin)), C.byref(run), m.ctypes.data_as(_dp),
+                              p.ctypes.data_as(_dp), sm.ctypes.data_as(_dp) if n_snap else None,
+                              sp.ctypes.data_as(_dp) if n_snap else None, pr.ctypes.data_as(_up))
+    if rc:
+        raise ValueError("oracle rejected the input")
+    return dict(m=m, p=p, snaps_m=sm[:n_snap], snaps_p=sp[:n_snap], promoter=pr)
+
+
+def cle_population(net, kin, pop, *, dt, T, cell_num, seed=0, net_id=0, init="steady"):
+    lib = load()
+    n_steps = int(np.ceil(T / dt))
+    s, keep = _net(net)
+    run = OrcRun(dt, n_steps, cell_num, 0, cell_num, seed & 0xFFFFFFFFFFFFFFFF, net_id,
+                 0 if init == "steady" else 1, 0, None, 1)
+    part = {"halve": 0, "binomial": 1}[pop.partition]
+    rep = {"uniform": 0, "exclude_mother": 1}[pop.replace]
+    op = OrcPopulation(1, pop.growth_rate, pop.V_div, pop.V_init[0], pop.V_init[1], pop.div_check_interval,
+                       part, rep)
+    G = net.G
+    m = np.empty((G, cell_num))
+    p = np.empty((G, cell_num))
+    V = np.empty(cell_num)
+    pr = np.empty((G, cell_num), dtype=np.uint8)
+    rc = lib.orc_cle_population(C.byref(s), C.byref(_kin(kin)), C.byref(run), C.byref(op),
+                                m.ctypes.data_as(_dp), p.ctypes.data_as(_dp), V.ctypes.data_as(_dp),
+                                pr.ctypes.data_as(_up))
+    if rc:
+        raise ValueError("oracle rejected the input")
+    return dict(m=m, p=p, V=V, promoter=pr)
+
+
+def binomial(n, p, c0, c1, c3, seed):
+    return load().orc_binomial(int(n), float(p), c0, c1, c3, seed & 0xFFFFFFFFFFFFFFFF)
+
+
+def capture(X_gc, efficiency, efficiency_std, seed, net_id=0):
+    """X_gc: (G, cells) array.  Returns (Y (G, cells) int32, p_cell (cells,))."""
+    lib = load()
+    X = np.ascontiguousarray(X_gc, dtype=np.float64)
+    G, cells = X.shape
+    Y = np.empty((G, cells), dtype=np.int32)
+    pc = np.empty(cells)
+    rc = lib.orc_capture(X.ctypes.data_as(_dp), cells, G, cells, efficiency, efficiency_std,
+                         seed & 0xFFFFFFFFFFFFFFFF, net_id, Y.ctypes.data_as(_ip), pc.ctypes.data_as(_dp))
+    if rc:
+        raise ValueError("oracle rejected the input")
+    return Y, pc
+
+
+def moments(m_gc, p_gc):
+    lib = load()
+    m = np.ascontiguousarray(m_gc, dtype=np.float64)
+    p = np.ascontiguousarray(p_gc, dtype=np.float64)
+    G, cells = m.shape
+    out = np.empty((G, 5))
+    lib.orc_moments(m.ctypes.data_as(_dp), p.ctypes.data_as(_dp), cells, G, cells, out.ctypes.data_as(_dp))
+    return out

@@ -1,0 +1,85 @@
+// Packed per-network descriptor ("blob") read by the CLE kernels.
+//
+// One blob per network, 32-bit words, 16-word aligned inside a single device
+// buffer.  Every workgroup serves one network, so each word is uniform across
+// the wave and is fetched through the scalar cache.
+//
+// Header words (floats stored by bit pattern):
+#pragma once
+#include <stdint.h>
+
+namespace scrna {
+
+enum BlobHeader : int {
+    H_G = 0,          // genes
+    H_S,              // source slots (genes whose Hill value some edge reads)
+    H_FLAGS,          // BlobFlags
+    H_NSTEPS,         // total steps n = ceil(T/dt)
+    H_SNAP_EVERY,     // steps between snapshots (0 = none)
+    H_NSNAP,          // number of snapshots
+    H_NETID,          // Philox stream id (counter word 3 = net_id<<4 | purpose)
+    H_CELLS,          // cells of this network
+    H_SEED_LO,
+    H_SEED_HI,
+    H_DT,             // f32 dt
+    H_NEG2LN2DT,      // f32 -2 ln(2) dt  (Box-Muller radius, scaled by sqrt(dt))
+    H_KT,             // f32 k_t
+    H_MUM,            // f32 effective mRNA decay
+    H_MUP,            // f32 effective protein decay
+    H_INVK,           // f32 1/K_d
+    H_HILLN,          // f32 Hill coefficient
+    H_SQRTDT,         // f32 sqrt(dt) (injected-noise scaling)
+    H_GROWTH,         // f32 exp(growth_rate*dt) (1 without population)
+    H_VDIV,           // f32 division volume
+    H_OFF_BETA,       // [G] f32 basal rates
+    H_OFF_SLOT,       // [G] i32 source slot of each gene, -1 if none
+    H_OFF_INOFF,      // [G+1] i32 CSR offsets (edge index) of additive in-edges per target
+    H_OFF_EDGES,      // [2*E] (lds_row = 2*slot + kind, f32 weight); kind 0 = act, 1 = rep
+    H_OFF_GATEOFF,    // [G+1] i32 word offsets (relative to H_OFF_GATES) of each target's gates
+    H_OFF_GATES,      // gate records: (type<<24 | nsrc), f32 strength*beta_t, nsrc slot words
+    H_OFF_TELE,       // [G][8] telegraph words, see TeleWord
+    H_OFF_INIT,       // [G][2] f32 initial m, p
+    H_OUT_FINAL_LO,   // element offset of [2][G][cells] f32 final state in the state pool
+    H_OUT_FINAL_HI,
+    H_OUT_SNAP_LO,    // element offset of [nsnap][2][G][cells] f32 snapshots in the snapshot pool
+    H_OUT_SNAP_HI,
+    H_OUT_PROM_LO,    // element offset of [G][cells] u8 promoter state in the promoter pool
+    H_OUT_PROM_HI,
+    H_OUT_V_LO,       // element offset of [cells] f32 volumes in the volume pool
+    H_OUT_V_HI,
+    H_INJ_LO,         // element offset of injected noise [n][G][2][cells] f32 (F_INJECT)
+    H_INJ_HI,
+    H_CH,             // genes per lane the network was compiled for
+    H_NW,             // waves per workgroup (NW*CH >= G)
+    H_WORDS = 48
+};
+
+enum BlobFlags : uint32_t {
+    F_TELE = 1u,      // some gene has a finite switching rate
+    F_HILL2 = 2u,     // Hill coefficient is exactly 2
+    F_GATES = 4u,     // AND / OR gates present
+    F_INJECT = 8u,    // read normals from the injected-noise buffer
+    F_POP = 16u       // population dynamics (volume-scaled transcription)
+};
+
+enum TeleWord : int {
+    T_FLAGS = 0,      // bit0: switches (isfinite k_on), bit1: random init (both finite)
+    T_ON_LO = 2,      // u64 threshold OFF->ON: switch when u32 < thr
+    T_ON_HI = 3,
+    T_OFF_LO = 4,     // u64 threshold ON->OFF
+    T_OFF_HI = 5,
+    T_INIT_LO = 6,    // u64 threshold for starting ON
+    T_INIT_HI = 7
+};
+
+enum GateType : uint32_t { GATE_AND = 0u, GATE_OR = 1u };
+
+// Tile of work: one workgroup = up to 64 cells of one network.
+struct Tile {
+    int32_t blob;        // word offset of the network's blob
+    int32_t cell0;       // first cell
+    int32_t ncell;       // cells in this tile (<= 64)
+    int32_t net;         // network index inside the plan
+};
+
+}  // namespace scrna

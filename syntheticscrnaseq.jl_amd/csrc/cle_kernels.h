@@ -1,0 +1,41 @@
+// Launch interface of the gfx950 CLE kernels (host C++ side).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "blob.h"
+
+namespace scrna {
+
+// One capture job = one cells x G matrix (element (c, g) at g*ld + c).
+struct CaptureJob {
+    uint64_t src_off;    // element offset into the source pool
+    uint64_t dst_off;    // element offset into the int32 count pool
+    int64_t src_ld, dst_ld;
+    int32_t cells, G;
+    double mu_log, sigma_log;   // log(eff) - sigma^2/2, sigma (capture.jl:30-31)
+    uint32_t k0, k1, net_id, pad;
+};
+
+// One moment job = one (m, p) pair of G x cells float matrices.
+struct MomentJob {
+    uint64_t m_off, p_off;      // element offsets into the float pool
+    uint64_t out_off;           // element offset into the double output ([G][5])
+    int32_t cells, G;
+};
+
+size_t stepper_lds_bytes(int S, int nbuf);
+hipError_t set_stepper_lds_limit(size_t bytes);
+hipError_t launch_stepper(int CH, int NW, int n_tiles, const uint32_t* blobs, const Tile* tiles,
+                          float* state_pool, float* snap_pool, uint8_t* prom_pool, float* vol_pool,
+                          const float* inj_pool, int s_begin, int s_end, int resume, int nbuf,
+                          size_t lds_bytes, hipStream_t stream);
+hipError_t launch_capture_f32(int n_jobs, int64_t max_elems, const CaptureJob* jobs, const float* src,
+                              int32_t* dst, hipStream_t stream);
+hipError_t launch_capture_f64(int n_jobs, int64_t max_elems, const CaptureJob* jobs, const double* src,
+                              int32_t* dst, hipStream_t stream);
+hipError_t launch_moments(int n_jobs, int max_G, const MomentJob* jobs, const float* pool, double* out,
+                          hipStream_t stream);
+
+}  // namespace scrna

@@ -1,0 +1,68 @@
+// Counter-based random numbers for the gfx950 CLE kernels.
+//
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC'11).  Key = 64-bit run seed; counter words:
+//   c0 = cell index inside its network, c1 = step (or gene / 0),
+//   c2 = sub-index (gene pair, gene quad, sampler block),
+//   c3 = (net_id << 4) | purpose.
+// Purposes are fixed below; the CPU oracle uses the same assignments, so it
+// can replay any cell's exact uniform stream.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace scrna {
+
+enum Purpose : uint32_t {
+    kNormal = 0,       // (cell, step, gene>>1): xi_m, xi_p for genes 2q and 2q+1
+    kSwitch = 1,       // (cell, step, gene>>2): one u32 per gene
+    kPromInit = 2,     // (cell, 0, gene>>2)
+    kCaptureEff = 3,   // (cell, 0, 0)
+    kCaptureBin = 4,   // (cell, gene, block)
+    kPartition = 5,    // (mother, step, ((2*gene+species)<<16) + block)
+    kReplace = 6,      // (mother, step, 0)
+    kVInit = 7         // (cell, 0, 0)
+};
+
+struct U4 {
+    uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                             uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return U4{c0, c1, c2, c3};
+}
+
+// Box-Muller on two u32.  The radius uniform ((a>>9)+1/2)*2^-23 and the
+// angle (b>>8)*2^-24 (revolutions) are exact in float, so the CPU oracle's
+// double transform sees the same uniforms.  neg2dt = -2*ln(2)*scale^2
+// folds the log2->ln conversion and a scale (sqrt(dt)) into the radius.
+__device__ __forceinline__ void normal_pair_scaled(uint32_t a, uint32_t b, float neg2ln2s2,
+                                                   float& z0, float& z1) {
+    const float u1 = ((float)(a >> 9) + 0.5f) * 0x1p-23f;
+    const float r = __builtin_amdgcn_sqrtf(neg2ln2s2 * __builtin_amdgcn_logf(u1));
+    const float th = (float)(b >> 8) * 0x1p-24f;   // revolutions in [0,1)
+    z0 = r * __builtin_amdgcn_cosf(th);             // cos(2*pi*th)
+    z1 = r * __builtin_amdgcn_sinf(th);             // sin(2*pi*th)
+}
+
+// 52-bit uniform in (0,1) from two u32 (capture and population draws).
+__device__ __forceinline__ double u52(uint32_t a, uint32_t b) {
+    const uint64_t x = ((uint64_t)a << 20) | (uint64_t)(b >> 12);
+    return ((double)x + 0.5) * 0x1p-52;
+}
+
+}  // namespace scrna

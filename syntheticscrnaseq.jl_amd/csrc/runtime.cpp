@@ -1,0 +1,856 @@
+// libscrna host runtime: C ABI (include/scrna.h), network compilation into
+// device blobs, HBM-budgeted plans, launches and downloads.
+//
+// Reference behaviour mirrored here:
+//   n_steps = ceil(T/dt) in double                  cle.jl:56, ext:69
+//   A_eff = A .* beta (rows), zero diagonal, split   network.jl:206-219
+//   mu_eff = mu + dilution (no population)           cle.jl:60-61
+//   intrinsic decay only with population             cle.jl:166-167
+//   init ceil(steady state)                          cle.jl:52-53, kinetics.jl:65-70
+//   telegraph init / switching probabilities         tauleap_binomial.jl:64-76, 110-121
+//   readout :mrna / :protein / :both                 cle.jl:123-129
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/scrna.h"
+#include "blob.h"
+#include "cle_kernels.h"
+#include "runtime_internal.h"
+
+namespace scrna {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                            \
+    do {                                                                                         \
+        hipError_t _e = (expr);                                                                  \
+        if (_e != hipSuccess)                                                                    \
+            return set_error(SCRNA_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(_e));  \
+    } while (0)
+
+static inline uint32_t f2w(float f) {
+    uint32_t w;
+    std::memcpy(&w, &f, 4);
+    return w;
+}
+
+// u = u32 * 2^-32 in [0,1); "u < q" <=> u32 < ceil(q * 2^32).
+static uint64_t prob_threshold(double q) {
+    if (!(q > 0.0)) return 0;
+    if (q >= 1.0) return 4294967296ull;
+    double t = std::ceil(q * 4294967296.0);
+    if (t > 4294967296.0) t = 4294967296.0;
+    return (uint64_t)t;
+}
+
+int64_t steps_for(double T, double dt) { return (int64_t)std::ceil(T / dt); }
+
+// ---------------------------------------------------------------------------
+// Context
+
+int ctx_alloc(scrna_ctx* ctx, void** p, size_t bytes) {
+    if (bytes == 0) {
+        *p = nullptr;
+        return SCRNA_OK;
+    }
+    size_t used = ctx->used.load();
+    if (used + bytes > ctx->budget)
+        return set_error(SCRNA_ERR_BUDGET, "HBM budget exceeded: need %zu more bytes, %zu of %zu in use",
+                         bytes, used, (size_t)ctx->budget);
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return set_error(SCRNA_ERR_DEVICE, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    ctx->used += bytes;
+    return SCRNA_OK;
+}
+
+void ctx_free(scrna_ctx* ctx, void* p, size_t bytes) {
+    if (!p) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipFree(p);
+    ctx->used -= bytes;
+}
+
+// ---------------------------------------------------------------------------
+// Network compilation
+
+int validate_inputs(const scrna_network* net, const scrna_kinetics* kin, const scrna_run* run,
+                    const scrna_population* pop, const scrna_capture* cap) {
+    if (!net || !kin || !run) return set_error(SCRNA_ERR_INVALID, "null network, kinetics or run");
+    const int G = net->G;
+    if (G <= 0) return set_error(SCRNA_ERR_INVALID, "G must be positive (got %d)", G);
+    if (G > kMaxGenes) return set_error(SCRNA_ERR_UNSUPPORTED, "G=%d exceeds the supported maximum %d", G, kMaxGenes);
+    if (!net->basals || !net->A) return set_error(SCRNA_ERR_INVALID, "basals and A are required");
+    for (int i = 0; i < G; ++i) {
+        if (!std::isfinite(net->basals[i])) return set_error(SCRNA_ERR_INVALID, "basal %d is not finite", i);
+        if (net->A[i + (size_t)i * G] != 0.0)   // network.jl:68
+            return set_error(SCRNA_ERR_INVALID, "interaction diagonal must be zero (A[%d,%d] != 0)", i, i);
+    }
+    for (size_t k = 0; k < (size_t)G * G; ++k)
+        if (!std::isfinite(net->A[k])) return set_error(SCRNA_ERR_INVALID, "interaction entry %zu is not finite", k);
+    auto check_gates = [&](int n, const int32_t* tgt, const int32_t* off, const int32_t* src, const double* str,
+                           const char* what) -> int {
+        if (n < 0) return set_error(SCRNA_ERR_INVALID, "negative %s gate count", what);
+        if (n == 0) return SCRNA_OK;
+        if (!tgt || !off || !src || !str) return set_error(SCRNA_ERR_INVALID, "%s gate arrays missing", what);
+        if (off[0] != 0) return set_error(SCRNA_ERR_INVALID, "%s gate offsets must start at 0", what);
+        for (int e = 0; e < n; ++e) {
+            if (tgt[e] < 0 || tgt[e] >= G) return set_error(SCRNA_ERR_INVALID, "%s gate %d target out of range", what, e);
+            if (off[e + 1] < off[e] || off[e + 1] - off[e] > 0xffff)
+                return set_error(SCRNA_ERR_INVALID, "%s gate %d has invalid source offsets", what, e);
+            for (int k = off[e]; k < off[e + 1]; ++k)
+                if (src[k] < 0 || src[k] >= G) return set_error(SCRNA_ERR_INVALID, "%s gate %d source out of range", what, e);
+            if (!std::isfinite(str[e])) return set_error(SCRNA_ERR_INVALID, "%s gate %d strength not finite", what, e);
+        }
+        return SCRNA_OK;
+    };
+    int rc = check_gates(net->n_coop, net->coop_target, net->coop_off, net->coop_src, net->coop_strength, "AND");
+    if (rc) return rc;
+    rc = check_gates(net->n_redun, net->redun_target, net->redun_off, net->redun_src, net->redun_strength, "OR");
+    if (rc) return rc;
+    if ((net->k_on == nullptr) != (net->k_off == nullptr))
+        return set_error(SCRNA_ERR_INVALID, "k_on and k_off must both be given or both be NULL");
+    if (net->k_on)
+        for (int g = 0; g < G; ++g)
+            if (std::isnan(net->k_on[g]) || std::isnan(net->k_off[g]) || net->k_on[g] < 0 || net->k_off[g] < 0)
+                return set_error(SCRNA_ERR_INVALID, "switching rates of gene %d must be >= 0 or Inf", g);
+    if (!(kin->K_d > 0.0) || !(kin->n > 0.0) || !std::isfinite(kin->K_d) || !std::isfinite(kin->n))
+        return set_error(SCRNA_ERR_INVALID, "K_d and n must be positive and finite");
+    if (!std::isfinite(kin->k_t) || !std::isfinite(kin->mu_m) || !std::isfinite(kin->mu_p) ||
+        !std::isfinite(kin->dilution) || kin->k_t < 0 || kin->mu_m < 0 || kin->mu_p < 0 || kin->dilution < 0)
+        return set_error(SCRNA_ERR_INVALID, "kinetic rates must be finite and non-negative");
+    if (!(run->dt > 0.0) || !std::isfinite(run->dt)) return set_error(SCRNA_ERR_INVALID, "dt must be positive");
+    if (!(run->T >= 0.0) || !std::isfinite(run->T)) return set_error(SCRNA_ERR_INVALID, "T must be finite and >= 0");
+    if (run->cell_num <= 0 || run->cell_num > (int64_t)INT32_MAX)
+        return set_error(SCRNA_ERR_INVALID, "cell_num must be in [1, 2^31)");
+    const int64_t n_steps = steps_for(run->T, run->dt);
+    if (n_steps > (int64_t)INT32_MAX) return set_error(SCRNA_ERR_INVALID, "too many steps (%lld)", (long long)n_steps);
+    if (run->readout < 0 || run->readout > 2) return set_error(SCRNA_ERR_INVALID, "readout must be 0, 1 or 2");
+    if (run->init_mode < 0 || run->init_mode > 1) return set_error(SCRNA_ERR_INVALID, "init_mode must be 0 or 1");
+    if (run->snapshot_every < 0) return set_error(SCRNA_ERR_INVALID, "snapshot_every must be >= 0");
+    if (run->precision < 0 || run->precision > 2) return set_error(SCRNA_ERR_INVALID, "unknown precision %d", run->precision);
+    if (run->net_id >= (1u << 28)) return set_error(SCRNA_ERR_INVALID, "net_id must be < 2^28");
+    if ((int64_t)G * run->cell_num > ((int64_t)1 << 40)) return set_error(SCRNA_ERR_INVALID, "problem too large");
+    if (pop && pop->enabled) {
+        if (!(pop->growth_rate >= 0.0) || !(pop->V_div > 0.0) || !(pop->V_init_lo > 0.0) ||
+            !(pop->V_init_hi >= pop->V_init_lo) || pop->div_check_interval <= 0)
+            return set_error(SCRNA_ERR_INVALID, "invalid population configuration");
+        if (pop->partition < 0 || pop->partition > 1 || pop->replace_mode < 0 || pop->replace_mode > 1)
+            return set_error(SCRNA_ERR_INVALID, "invalid population partition / replace mode");
+    }
+    if (cap && cap->enabled) {
+        // CaptureModel asserts readout in (:mrna, :protein) (population.jl:63)
+        if (cap->readout != SCRNA_READOUT_MRNA && cap->readout != SCRNA_READOUT_PROTEIN)
+            return set_error(SCRNA_ERR_INVALID, "capture readout must be mrna or protein");
+        if (!(cap->efficiency > 0.0) || !std::isfinite(cap->efficiency) || !(cap->efficiency_std >= 0.0) ||
+            !std::isfinite(cap->efficiency_std))
+            return set_error(SCRNA_ERR_INVALID, "capture efficiency must be > 0 and efficiency_std >= 0");
+    }
+    return SCRNA_OK;
+}
+
+// Genes per lane: the smallest chunk that keeps NW = ceil(G/CH) <= 16 waves.
+static void choose_shape(int G, int* CH, int* NW) {
+    int ch = 2;
+    if (G > 32) ch = 4;
+    if (G > 64) ch = 8;
+    if (G > 128) ch = 16;
+    *CH = ch;
+    *NW = (G + ch - 1) / ch;
+}
+
+int compile_network(const scrna_network* net, const scrna_kinetics* kin, const scrna_run* run,
+                    const scrna_population* pop, int force_ch, CompiledNet* out) {
+    const int G = net->G;
+    const bool population = pop && pop->enabled;
+    out->G = G;
+    out->cells = run->cell_num;
+    out->n_steps = steps_for(run->T, run->dt);
+    out->snap_every = run->snapshot_every;
+    out->n_snap = run->snapshot_every > 0 ? out->n_steps / run->snapshot_every : 0;
+    out->population = population;
+    if (population) out->pop = *pop;
+    int CH, NW;
+    choose_shape(G, &CH, &NW);
+    if (force_ch == 2 || force_ch == 4 || force_ch == 8 || force_ch == 16) {
+        CH = force_ch;
+        NW = (G + CH - 1) / CH;
+        if (NW > 16) return set_error(SCRNA_ERR_INVALID, "tile override CH=%d gives %d waves > 16", CH, NW);
+    }
+    out->CH = CH;
+    out->NW = NW;
+
+    // A_eff split (network.jl:206-219), as per-target CSR over source genes.
+    std::vector<int> is_src(G, 0);
+    std::vector<std::vector<std::pair<int, double>>> in_edges(G);
+    for (int i = 0; i < G; ++i)
+        for (int j = 0; j < G; ++j) {
+            if (i == j) continue;
+            const double aeff = net->A[i + (size_t)j * G] * net->basals[i];
+            if (aeff != 0.0) {
+                in_edges[i].push_back({j, aeff});
+                is_src[j] = 1;
+            }
+        }
+    struct GateRec {
+        uint32_t type;
+        std::vector<int> src;
+        double strength_beta;
+    };
+    std::vector<std::vector<GateRec>> gates_of(G);
+    for (int e = 0; e < net->n_coop; ++e) {
+        GateRec r{GATE_AND, {}, net->coop_strength[e] * net->basals[net->coop_target[e]]};
+        for (int k = net->coop_off[e]; k < net->coop_off[e + 1]; ++k) {
+            r.src.push_back(net->coop_src[k]);
+            is_src[net->coop_src[k]] = 1;
+        }
+        gates_of[net->coop_target[e]].push_back(r);
+    }
+    for (int e = 0; e < net->n_redun; ++e) {
+        GateRec r{GATE_OR, {}, net->redun_strength[e] * net->basals[net->redun_target[e]]};
+        for (int k = net->redun_off[e]; k < net->redun_off[e + 1]; ++k) {
+            r.src.push_back(net->redun_src[k]);
+            is_src[net->redun_src[k]] = 1;
+        }
+        gates_of[net->redun_target[e]].push_back(r);
+    }
+    std::vector<int32_t> slot_of(G, -1);
+    int S = 0;
+    for (int g = 0; g < G; ++g)
+        if (is_src[g]) slot_of[g] = S++;
+    out->S = S;
+    const bool any_gates = net->n_coop > 0 || net->n_redun > 0;
+
+    bool tele = false;
+    if (net->k_on)
+        for (int g = 0; g < G; ++g)
+            if (std::isfinite(net->k_on[g]) || std::isfinite(net->k_off[g])) tele = true;   // network.jl:103
+
+    std::vector<uint32_t>& b = out->blob;
+    b.assign(H_WORDS, 0u);
+    auto align16 = [&]() {
+        while (b.size() % 16) b.push_back(0u);
+    };
+    const double mu_m = population ? kin->mu_m : kin->mu_m + kin->dilution;
+    const double mu_p = population ? kin->mu_p : kin->mu_p + kin->dilution;
+    uint32_t flags = 0;
+    if (tele) flags |= F_TELE;
+    if (kin->n == 2.0) flags |= F_HILL2;
+    if (any_gates) flags |= F_GATES;
+    if (run->injected_noise) flags |= F_INJECT;
+    if (population) flags |= F_POP;
+    b[H_G] = (uint32_t)G;
+    b[H_S] = (uint32_t)S;
+    b[H_FLAGS] = flags;
+    b[H_NSTEPS] = (uint32_t)out->n_steps;
+    b[H_SNAP_EVERY] = (uint32_t)run->snapshot_every;
+    b[H_NSNAP] = (uint32_t)out->n_snap;
+    b[H_NETID] = run->net_id;
+    b[H_CELLS] = (uint32_t)run->cell_num;
+    b[H_SEED_LO] = (uint32_t)run->seed;
+    b[H_SEED_HI] = (uint32_t)(run->seed >> 32);
+    b[H_DT] = f2w((float)run->dt);
+    b[H_NEG2LN2DT] = f2w((float)(-2.0 * std::log(2.0) * run->dt));
+    b[H_KT] = f2w((float)kin->k_t);
+    b[H_MUM] = f2w((float)mu_m);
+    b[H_MUP] = f2w((float)mu_p);
+    b[H_INVK] = f2w((float)(1.0 / kin->K_d));
+    b[H_HILLN] = f2w((float)kin->n);
+    b[H_SQRTDT] = f2w((float)std::sqrt(run->dt));
+    b[H_GROWTH] = f2w(population ? (float)std::exp(pop->growth_rate * run->dt) : 1.0f);
+    b[H_VDIV] = f2w(population ? (float)pop->V_div : 0.0f);
+    b[H_CH] = (uint32_t)CH;
+    b[H_NW] = (uint32_t)NW;
+
+    b[H_OFF_BETA] = (uint32_t)b.size();
+    for (int g = 0; g < G; ++g) b.push_back(f2w((float)net->basals[g]));
+    align16();
+    b[H_OFF_SLOT] = (uint32_t)b.size();
+    for (int g = 0; g < G; ++g) b.push_back((uint32_t)slot_of[g]);
+    align16();
+    b[H_OFF_INOFF] = (uint32_t)b.size();
+    {
+        int e = 0;
+        for (int g = 0; g < G; ++g) {
+            b.push_back((uint32_t)e);
+            e += (int)in_edges[g].size();
+        }
+        b.push_back((uint32_t)e);
+        out->n_edges = e;
+    }
+    align16();
+    b[H_OFF_EDGES] = (uint32_t)b.size();
+    for (int g = 0; g < G; ++g)
+        for (auto& ed : in_edges[g]) {
+            const int kind = ed.second > 0.0 ? 0 : 1;   // A_pos reads act, A_neg reads rep
+            b.push_back((uint32_t)(2 * slot_of[ed.first] + kind));
+            b.push_back(f2w((float)std::fabs(ed.second)));
+        }
+    align16();
+    b[H_OFF_GATEOFF] = (uint32_t)b.size();
+    {
+        uint32_t w = 0;
+        for (int g = 0; g < G; ++g) {
+            b.push_back(w);
+            for (auto& r : gates_of[g]) w += 2 + (uint32_t)r.src.size();
+        }
+        b.push_back(w);
+    }
+    align16();
+    b[H_OFF_GATES] = (uint32_t)b.size();
+    for (int g = 0; g < G; ++g)
+        for (auto& r : gates_of[g]) {
+            b.push_back((r.type << 24) | (uint32_t)r.src.size());
+            b.push_back(f2w((float)r.strength_beta));
+            for (int s : r.src) b.push_back((uint32_t)slot_of[s]);
+        }
+    align16();
+    b[H_OFF_TELE] = (uint32_t)b.size();
+    for (int g = 0; g < G; ++g) {
+        uint32_t tf = 0;
+        uint64_t thr_on = 0, thr_off = 0, thr_init = 0;
+        if (net->k_on) {
+            const double kon = net->k_on[g], koff = net->k_off[g];
+            if (std::isfinite(kon)) {                                     // tauleap_binomial.jl:112
+                tf |= 1u;
+                thr_on = prob_threshold(1.0 - std::exp(-kon * run->dt));  // tauleap_binomial.jl:118
+                thr_off = prob_threshold(1.0 - std::exp(-koff * run->dt));// tauleap_binomial.jl:113
+            }
+            if (std::isfinite(kon) && std::isfinite(koff)) {              // tauleap_binomial.jl:68-70
+                tf |= 2u;
+                thr_init = prob_threshold(kon / (kon + koff));
+            }
+        }
+        b.push_back(tf);
+        b.push_back(0u);
+        b.push_back((uint32_t)thr_on);
+        b.push_back((uint32_t)(thr_on >> 32));
+        b.push_back((uint32_t)thr_off);
+        b.push_back((uint32_t)(thr_off >> 32));
+        b.push_back((uint32_t)thr_init);
+        b.push_back((uint32_t)(thr_init >> 32));
+    }
+    align16();
+    b[H_OFF_INIT] = (uint32_t)b.size();
+    for (int g = 0; g < G; ++g) {
+        const double mss = net->basals[g] / (kin->mu_m + kin->dilution);   // kinetics.jl:67
+        const double pss = kin->k_t * mss / (kin->mu_p + kin->dilution);   // kinetics.jl:68
+        const float m0 = run->init_mode == 0 ? (float)std::ceil(mss) : 0.0f;
+        const float p0 = run->init_mode == 0 ? (float)std::ceil(pss) : 0.0f;
+        b.push_back(f2w(m0));
+        b.push_back(f2w(p0));
+    }
+    align16();
+    return SCRNA_OK;
+}
+
+static void put_u64(std::vector<uint32_t>& b, int idx, uint64_t v) {
+    b[idx] = (uint32_t)v;
+    b[idx + 1] = (uint32_t)(v >> 32);
+}
+
+// ---------------------------------------------------------------------------
+// Plans
+
+void plan_release(scrna_plan* p) {
+    if (!p) return;
+    scrna_ctx* ctx = p->ctx;
+    for (auto& a : p->allocs) ctx_free(ctx, a.first, a.second);
+    p->allocs.clear();
+    if (p->ev_start) (void)hipEventDestroy(p->ev_start);
+    if (p->ev_stop) (void)hipEventDestroy(p->ev_stop);
+    p->ev_start = p->ev_stop = nullptr;
+}
+
+int plan_alloc(scrna_plan* p, void** ptr, size_t bytes) {
+    int rc = ctx_alloc(p->ctx, ptr, bytes);
+    if (rc) return rc;
+    if (*ptr) p->allocs.push_back({*ptr, bytes});
+    p->bytes += bytes;
+    return SCRNA_OK;
+}
+
+int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna_kinetics* kins,
+               const scrna_run* runs, const scrna_population* pops, const scrna_capture* caps,
+               int force_ch, scrna_plan** out) {
+    if (!ctx || !out || n <= 0 || !nets || !kins || !runs)
+        return set_error(SCRNA_ERR_INVALID, "plan needs a context, n > 0 and network/kinetics/run arrays");
+    for (int i = 0; i < n; ++i) {
+        int rc = validate_inputs(&nets[i], &kins[i], &runs[i], pops ? &pops[i] : nullptr, caps ? &caps[i] : nullptr);
+        if (rc) return set_error(rc, "network %d: %s", i, g_last_error.c_str());
+        if (runs[i].precision != SCRNA_PREC_FP32 && !(nets[i].G >= 16))
+            return set_error(SCRNA_ERR_INVALID, "network %d: MFMA precision modes need G >= 16", i);
+    }
+    std::unique_ptr<scrna_plan> P(new scrna_plan());
+    P->ctx = ctx;
+    P->nets.resize(n);
+    std::vector<uint32_t> blobs;
+    size_t state_elems = 0, snap_elems = 0, prom_elems = 0, vol_elems = 0, inj_elems = 0, count_elems = 0,
+           mom_elems = 0;
+    for (int i = 0; i < n; ++i) {
+        CompiledNet& cn = P->nets[i];
+        int rc = compile_network(&nets[i], &kins[i], &runs[i], pops ? &pops[i] : nullptr, force_ch, &cn);
+        if (rc) return rc;
+        cn.readout = runs[i].readout;
+        cn.precision = runs[i].precision;
+        cn.capture = caps && caps[i].enabled;
+        if (cn.capture) cn.cap = caps[i];
+        cn.seed = runs[i].seed;
+        cn.net_id = runs[i].net_id;
+        const size_t GC = (size_t)cn.G * (size_t)cn.cells;
+        cn.off_state = state_elems;
+        state_elems += 2 * GC;
+        cn.off_snap = snap_elems;
+        snap_elems += (size_t)cn.n_snap * 2 * GC;
+        cn.off_prom = prom_elems;
+        prom_elems += GC;
+        cn.off_vol = vol_elems;
+        if (cn.population) vol_elems += (size_t)cn.cells;
+        cn.off_inj = inj_elems;
+        if (runs[i].injected_noise) inj_elems += (size_t)cn.n_steps * 2 * GC;
+        cn.off_counts = count_elems;
+        if (cn.capture) count_elems += GC;
+        cn.n_mom = cn.n_snap > 0 ? cn.n_snap : 1;
+        cn.off_mom = mom_elems;
+        mom_elems += (size_t)cn.n_mom * cn.G * 5;
+        put_u64(cn.blob, H_OUT_FINAL_LO, cn.off_state);
+        put_u64(cn.blob, H_OUT_SNAP_LO, cn.off_snap);
+        put_u64(cn.blob, H_OUT_PROM_LO, cn.off_prom);
+        put_u64(cn.blob, H_OUT_V_LO, cn.off_vol);
+        put_u64(cn.blob, H_INJ_LO, cn.off_inj);
+        cn.blob_off = blobs.size();
+        blobs.insert(blobs.end(), cn.blob.begin(), cn.blob.end());
+        if (blobs.size() > (size_t)INT32_MAX) return set_error(SCRNA_ERR_INVALID, "network descriptors too large");
+        if (cn.population) P->has_population = true;
+    }
+    if (P->has_population) {
+        for (int i = 0; i < n; ++i)
+            if (!P->nets[i].population)
+                return set_error(SCRNA_ERR_UNSUPPORTED, "a plan mixes population and non-population networks");
+    }
+
+    // Launch groups: networks sharing (CH, NW); tiles of 64 cells.
+    std::map<std::pair<int, int>, std::vector<Tile>> groups;
+    std::map<std::pair<int, int>, int> group_S;
+    for (int i = 0; i < n; ++i) {
+        const CompiledNet& cn = P->nets[i];
+        auto key = std::make_pair(cn.CH, cn.NW);
+        auto& v = groups[key];
+        int& s = group_S[key];
+        s = std::max(s, cn.S);
+        for (int64_t c0 = 0; c0 < cn.cells; c0 += 64) {
+            Tile t;
+            t.blob = (int32_t)cn.blob_off;
+            t.cell0 = (int32_t)c0;
+            t.ncell = (int32_t)std::min<int64_t>(64, cn.cells - c0);
+            t.net = i;
+            v.push_back(t);
+        }
+    }
+    std::vector<Tile> all_tiles;
+    for (auto& kv : groups) {
+        LaunchGroup lg;
+        lg.CH = kv.first.first;
+        lg.NW = kv.first.second;
+        lg.tile0 = (int)all_tiles.size();
+        lg.n_tiles = (int)kv.second.size();
+        const int S = std::max(1, group_S[kv.first]);
+        lg.nbuf = stepper_lds_bytes(S, 2) <= ctx->lds_limit_soft ? 2 : 1;
+        lg.lds = stepper_lds_bytes(S, lg.nbuf);
+        if (lg.lds > ctx->lds_max)
+            return set_error(SCRNA_ERR_UNSUPPORTED, "%d source genes need %zu B of LDS per workgroup (max %zu)", S,
+                             lg.lds, ctx->lds_max);
+        all_tiles.insert(all_tiles.end(), kv.second.begin(), kv.second.end());
+        P->groups.push_back(lg);
+    }
+
+    // Capture and moment jobs.
+    std::vector<CaptureJob> cjobs;
+    std::vector<MomentJob> mjobs;
+    for (int i = 0; i < n; ++i) {
+        const CompiledNet& cn = P->nets[i];
+        const size_t GC = (size_t)cn.G * (size_t)cn.cells;
+        if (cn.capture) {
+            CaptureJob j{};
+            j.src_off = cn.off_state + (cn.cap.readout == SCRNA_READOUT_PROTEIN ? GC : 0);
+            j.dst_off = cn.off_counts;
+            j.src_ld = cn.cells;
+            j.dst_ld = cn.cells;
+            j.cells = (int32_t)cn.cells;
+            j.G = cn.G;
+            j.mu_log = std::log(cn.cap.efficiency) - cn.cap.efficiency_std * cn.cap.efficiency_std / 2.0;
+            j.sigma_log = cn.cap.efficiency_std;
+            j.k0 = (uint32_t)cn.seed;
+            j.k1 = (uint32_t)(cn.seed >> 32);
+            j.net_id = cn.net_id;
+            cjobs.push_back(j);
+            P->max_cap_elems = std::max<int64_t>(P->max_cap_elems, (int64_t)GC);
+        }
+        for (int k = 0; k < cn.n_mom; ++k) {
+            MomentJob j{};
+            if (cn.n_snap > 0) {
+                j.m_off = cn.off_snap + (size_t)(2 * k) * GC;
+                j.p_off = cn.off_snap + (size_t)(2 * k + 1) * GC;
+                j.m_off |= kSnapPoolBit;
+                j.p_off |= kSnapPoolBit;
+            } else {
+                j.m_off = cn.off_state;
+                j.p_off = cn.off_state + GC;
+            }
+            j.out_off = cn.off_mom + (size_t)k * cn.G * 5;
+            j.cells = (int32_t)cn.cells;
+            j.G = cn.G;
+            mjobs.push_back(j);
+            P->max_mom_G = std::max(P->max_mom_G, cn.G);
+        }
+    }
+    // Moment jobs read either pool; split them by pool.
+    for (auto& j : mjobs) {
+        if (j.m_off & kSnapPoolBit) {
+            j.m_off &= ~kSnapPoolBit;
+            j.p_off &= ~kSnapPoolBit;
+            P->mjobs_snap.push_back(j);
+        } else {
+            P->mjobs_state.push_back(j);
+        }
+    }
+
+    // Device allocations.
+    int rc;
+    void* ptr;
+#define PLAN_ALLOC(field, type, count)                                 \
+    do {                                                               \
+        rc = plan_alloc(P.get(), &ptr, sizeof(type) * (size_t)(count)); \
+        if (rc) {                                                      \
+            plan_release(P.get());                                     \
+            return rc;                                                 \
+        }                                                              \
+        P->field = (type*)ptr;                                         \
+    } while (0)
+    PLAN_ALLOC(d_blobs, uint32_t, blobs.size());
+    PLAN_ALLOC(d_tiles, Tile, all_tiles.size());
+    PLAN_ALLOC(d_state, float, state_elems);
+    PLAN_ALLOC(d_snap, float, snap_elems);
+    PLAN_ALLOC(d_prom, uint8_t, prom_elems);
+    PLAN_ALLOC(d_vol, float, vol_elems);
+    PLAN_ALLOC(d_inj, float, inj_elems);
+    PLAN_ALLOC(d_counts, int32_t, count_elems);
+    PLAN_ALLOC(d_mom, double, mom_elems);
+    PLAN_ALLOC(d_cjobs, CaptureJob, cjobs.size());
+    PLAN_ALLOC(d_mjobs_snap, MomentJob, P->mjobs_snap.size());
+    PLAN_ALLOC(d_mjobs_state, MomentJob, P->mjobs_state.size());
+#undef PLAN_ALLOC
+    P->n_cjobs = (int)cjobs.size();
+    hipStream_t st = ctx->stream;
+    auto up = [&](void* dst, const void* src, size_t bytes) -> int {
+        if (bytes == 0) return SCRNA_OK;
+        hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) return set_error(SCRNA_ERR_DEVICE, "upload failed: %s", hipGetErrorString(e));
+        return SCRNA_OK;
+    };
+    if ((rc = up(P->d_blobs, blobs.data(), blobs.size() * 4)) ||
+        (rc = up(P->d_tiles, all_tiles.data(), all_tiles.size() * sizeof(Tile))) ||
+        (rc = up(P->d_cjobs, cjobs.data(), cjobs.size() * sizeof(CaptureJob))) ||
+        (rc = up(P->d_mjobs_snap, P->mjobs_snap.data(), P->mjobs_snap.size() * sizeof(MomentJob))) ||
+        (rc = up(P->d_mjobs_state, P->mjobs_state.data(), P->mjobs_state.size() * sizeof(MomentJob)))) {
+        plan_release(P.get());
+        return rc;
+    }
+    for (int i = 0; i < n; ++i) {
+        if (!runs[i].injected_noise) continue;
+        const CompiledNet& cn = P->nets[i];
+        if ((rc = up(P->d_inj + cn.off_inj, runs[i].injected_noise,
+                     (size_t)cn.n_steps * 2 * cn.G * cn.cells * sizeof(float)))) {
+            plan_release(P.get());
+            return rc;
+        }
+    }
+    hipError_t e = hipStreamSynchronize(st);   // host arrays may be freed after return
+    if (e != hipSuccess) {
+        plan_release(P.get());
+        return set_error(SCRNA_ERR_DEVICE, "upload sync failed: %s", hipGetErrorString(e));
+    }
+    if (hipEventCreate(&P->ev_start) != hipSuccess || hipEventCreate(&P->ev_stop) != hipSuccess) {
+        plan_release(P.get());
+        return set_error(SCRNA_ERR_DEVICE, "hipEventCreate failed");
+    }
+    for (auto& cn : P->nets) P->cell_gene_steps += (double)cn.G * (double)cn.cells * (double)cn.n_steps;
+    *out = P.release();
+    return SCRNA_OK;
+}
+
+int plan_launch(scrna_plan* P, hipStream_t st) {
+    if (!st) st = P->ctx->stream;
+    HIP_TRY(hipSetDevice(P->ctx->device));
+    HIP_TRY(hipEventRecord(P->ev_start, st));
+    if (P->has_population) {
+        int rc = population_launch(P, st);
+        if (rc) return rc;
+    } else {
+        for (const LaunchGroup& lg : P->groups)
+            HIP_TRY(launch_stepper(lg.CH, lg.NW, lg.n_tiles, P->d_blobs, P->d_tiles + lg.tile0, P->d_state,
+                                   P->d_snap, P->d_prom, P->d_vol, P->d_inj, 0, INT32_MAX, 0, lg.nbuf, lg.lds, st));
+    }
+    HIP_TRY(hipEventRecord(P->ev_stop, st));
+    HIP_TRY(launch_moments((int)P->mjobs_snap.size(), P->max_mom_G, P->d_mjobs_snap, P->d_snap, P->d_mom, st));
+    HIP_TRY(launch_moments((int)P->mjobs_state.size(), P->max_mom_G, P->d_mjobs_state, P->d_state, P->d_mom, st));
+    HIP_TRY(launch_capture_f32(P->n_cjobs, P->max_cap_elems, P->d_cjobs, P->d_state, P->d_counts, st));
+    P->launched = true;
+    return SCRNA_OK;
+}
+
+int plan_download(scrna_plan* P, int32_t i, scrna_output* out) {
+    if (i < 0 || i >= (int32_t)P->nets.size()) return set_error(SCRNA_ERR_INVALID, "network index %d out of range", i);
+    if (!out) return set_error(SCRNA_ERR_INVALID, "null output");
+    const CompiledNet& cn = P->nets[i];
+    const size_t C = (size_t)cn.cells, G = (size_t)cn.G, GC = G * C;
+    HIP_TRY(hipSetDevice(P->ctx->device));
+    if (out->y) {
+        const int64_t ld = out->ld > 0 ? out->ld : (int64_t)C;
+        if (ld < (int64_t)C) return set_error(SCRNA_ERR_INVALID, "output leading dimension %lld < cells", (long long)ld);
+        std::vector<float> h(2 * GC);
+        HIP_TRY(hipMemcpy(h.data(), P->d_state + cn.off_state, 2 * GC * sizeof(float), hipMemcpyDeviceToHost));
+        // cle.jl:123-129 -- columns are genes; :both puts mRNA first.
+        const int ncol = cn.readout == SCRNA_READOUT_BOTH ? 2 * (int)G : (int)G;
+        for (int col = 0; col < ncol; ++col) {
+            size_t src_row;
+            if (cn.readout == SCRNA_READOUT_MRNA) src_row = col;
+            else if (cn.readout == SCRNA_READOUT_PROTEIN) src_row = G + col;
+            else src_row = col;   // both: rows 0..G-1 mRNA, G..2G-1 protein
+            const float* s = h.data() + src_row * C;
+            double* d = out->y + (size_t)col * ld;
+            for (size_t c = 0; c < C; ++c) d[c] = (double)s[c];
+        }
+    }
+    if (out->counts) {
+        if (!cn.capture) return set_error(SCRNA_ERR_INVALID, "network %d was run without capture", i);
+        const int64_t ld = out->ld > 0 ? out->ld : (int64_t)C;
+        if (ld == (int64_t)C) {
+            HIP_TRY(hipMemcpy(out->counts, P->d_counts + cn.off_counts, GC * sizeof(int32_t), hipMemcpyDeviceToHost));
+        } else {
+            for (size_t g = 0; g < G; ++g)
+                HIP_TRY(hipMemcpy(out->counts + g * ld, P->d_counts + cn.off_counts + g * C, C * sizeof(int32_t),
+                                  hipMemcpyDeviceToHost));
+        }
+    }
+    if (out->snapshots && cn.n_snap > 0)
+        HIP_TRY(hipMemcpy(out->snapshots, P->d_snap + cn.off_snap, (size_t)cn.n_snap * 2 * GC * sizeof(float),
+                          hipMemcpyDeviceToHost));
+    if (out->moments)
+        HIP_TRY(hipMemcpy(out->moments, P->d_mom + cn.off_mom, (size_t)cn.n_mom * G * 5 * sizeof(double),
+                          hipMemcpyDeviceToHost));
+    if (out->volumes && cn.population) {
+        std::vector<float> v(C);
+        HIP_TRY(hipMemcpy(v.data(), P->d_vol + cn.off_vol, C * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t c = 0; c < C; ++c) out->volumes[c] = (double)v[c];
+    }
+    return SCRNA_OK;
+}
+
+}  // namespace scrna
+
+// ===========================================================================
+// C ABI
+
+using namespace scrna;
+
+extern "C" {
+
+int32_t scrna_abi_version(void) { return SCRNA_ABI_VERSION; }
+
+const char* scrna_version(void) { return "libscrna 0.1.0 (gfx950)"; }
+
+const char* scrna_last_error(void) { return g_last_error.c_str(); }
+
+int scrna_ctx_create(int32_t device, uint64_t hbm_budget_bytes, scrna_ctx** out) {
+    if (!out) return set_error(SCRNA_ERR_INVALID, "null output pointer");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0)
+        return set_error(SCRNA_ERR_DEVICE, "no HIP device available (%s)", hipGetErrorString(e));
+    if (device < 0 || device >= count) return set_error(SCRNA_ERR_INVALID, "device %d out of range [0,%d)", device, count);
+    HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return set_error(SCRNA_ERR_DEVICE, "libscrna is built for gfx950, device %d is %s", device, prop.gcnArchName);
+    std::unique_ptr<scrna_ctx> C(new scrna_ctx());
+    C->device = device;
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    C->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)(0.9 * (double)free_b);
+    int lds = 0;
+    HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    C->lds_max = (size_t)lds;
+    C->lds_limit_soft = std::min<size_t>(C->lds_max, 64 * 1024);
+    C->num_cu = prop.multiProcessorCount;
+    if (C->lds_max > 65536 && set_stepper_lds_limit(C->lds_max) != hipSuccess) {
+        (void)hipGetLastError();
+        C->lds_max = 65536;   // large dynamic LDS not granted: stay within the default limit
+    }
+    C->lds_limit_soft = std::min(C->lds_limit_soft, C->lds_max);
+    HIP_TRY(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
+    *out = C.release();
+    return SCRNA_OK;
+}
+
+int scrna_ctx_destroy(scrna_ctx* ctx) {
+    if (!ctx) return SCRNA_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+    return SCRNA_OK;
+}
+
+int scrna_plan_create(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna_kinetics* kins,
+                      const scrna_run* runs, const scrna_population* pops, const scrna_capture* captures,
+                      scrna_plan** out) {
+    if (!out) return set_error(SCRNA_ERR_INVALID, "null output pointer");
+    *out = nullptr;
+    if (!ctx) return set_error(SCRNA_ERR_INVALID, "null context");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return plan_build(ctx, n, nets, kins, runs, pops, captures, ctx->force_ch, out);
+}
+
+int scrna_plan_launch(scrna_plan* plan, void* stream) {
+    if (!plan) return set_error(SCRNA_ERR_INVALID, "null plan");
+    return plan_launch(plan, (hipStream_t)stream);
+}
+
+int scrna_plan_sync(scrna_plan* plan) {
+    if (!plan) return set_error(SCRNA_ERR_INVALID, "null plan");
+    HIP_TRY(hipSetDevice(plan->ctx->device));
+    HIP_TRY(hipEventSynchronize(plan->ev_stop));
+    HIP_TRY(hipDeviceSynchronize());
+    return SCRNA_OK;
+}
+
+int scrna_plan_download(scrna_plan* plan, int32_t i, scrna_output* out) {
+    if (!plan) return set_error(SCRNA_ERR_INVALID, "null plan");
+    return plan_download(plan, i, out);
+}
+
+int scrna_plan_stepper_ms(scrna_plan* plan, double* ms, double* cgs) {
+    if (!plan || !ms) return set_error(SCRNA_ERR_INVALID, "null argument");
+    if (!plan->launched) return set_error(SCRNA_ERR_INVALID, "plan has not been launched");
+    float f = 0.0f;
+    HIP_TRY(hipEventSynchronize(plan->ev_stop));
+    HIP_TRY(hipEventElapsedTime(&f, plan->ev_start, plan->ev_stop));
+    *ms = (double)f;
+    if (cgs) *cgs = plan->cell_gene_steps;
+    return SCRNA_OK;
+}
+
+int scrna_plan_bytes(scrna_plan* plan, uint64_t* bytes) {
+    if (!plan || !bytes) return set_error(SCRNA_ERR_INVALID, "null argument");
+    *bytes = plan->bytes;
+    return SCRNA_OK;
+}
+
+int scrna_plan_destroy(scrna_plan* plan) {
+    if (!plan) return SCRNA_OK;
+    (void)hipSetDevice(plan->ctx->device);
+    (void)hipDeviceSynchronize();
+    plan_release(plan);
+    delete plan;
+    return SCRNA_OK;
+}
+
+int scrna_simulate_cle_batch(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna_kinetics* kins,
+                             const scrna_run* runs, const scrna_population* pops, const scrna_capture* captures,
+                             scrna_output* outs) {
+    if (!ctx) return set_error(SCRNA_ERR_INVALID, "null context");
+    if (!outs) return set_error(SCRNA_ERR_INVALID, "null outputs");
+    scrna_plan* plan = nullptr;
+    int rc = scrna_plan_create(ctx, n, nets, kins, runs, pops, captures, &plan);
+    if (rc) return rc;
+    rc = scrna_plan_launch(plan, nullptr);
+    if (!rc) rc = scrna_plan_sync(plan);
+    for (int i = 0; !rc && i < n; ++i) rc = scrna_plan_download(plan, i, &outs[i]);
+    std::string err = rc ? g_last_error : std::string();
+    scrna_plan_destroy(plan);
+    if (rc) g_last_error = err;
+    return rc;
+}
+
+int scrna_simulate_cle(scrna_ctx* ctx, const scrna_network* net, const scrna_kinetics* kin, const scrna_run* run,
+                       const scrna_population* pop, const scrna_capture* capture, scrna_output* out) {
+    return scrna_simulate_cle_batch(ctx, 1, net, kin, run, pop, capture, out);
+}
+
+int scrna_apply_capture(scrna_ctx* ctx, const double* X, int64_t cells, int32_t G, int64_t ld, double efficiency,
+                        double efficiency_std, uint64_t seed, uint32_t net_id, int32_t* Y) {
+    if (!ctx || !X || !Y) return set_error(SCRNA_ERR_INVALID, "null argument");
+    if (cells <= 0 || G <= 0 || ld < cells || cells > INT32_MAX)
+        return set_error(SCRNA_ERR_INVALID, "invalid shape cells=%lld G=%d ld=%lld", (long long)cells, G, (long long)ld);
+    if (!(efficiency > 0.0) || !(efficiency_std >= 0.0) || !std::isfinite(efficiency) || !std::isfinite(efficiency_std))
+        return set_error(SCRNA_ERR_INVALID, "capture efficiency must be > 0 and efficiency_std >= 0");
+    if (net_id >= (1u << 28)) return set_error(SCRNA_ERR_INVALID, "net_id must be < 2^28");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const size_t GC = (size_t)G * (size_t)cells;
+    void *dX = nullptr, *dY = nullptr, *dJ = nullptr;
+    int rc = ctx_alloc(ctx, &dX, GC * sizeof(double));
+    if (!rc) rc = ctx_alloc(ctx, &dY, GC * sizeof(int32_t));
+    if (!rc) rc = ctx_alloc(ctx, &dJ, sizeof(CaptureJob));
+    auto cleanup = [&]() {
+        ctx_free(ctx, dX, GC * sizeof(double));
+        ctx_free(ctx, dY, GC * sizeof(int32_t));
+        ctx_free(ctx, dJ, sizeof(CaptureJob));
+    };
+    if (rc) {
+        cleanup();
+        return rc;
+    }
+    CaptureJob j{};
+    j.src_off = 0;
+    j.dst_off = 0;
+    j.src_ld = cells;
+    j.dst_ld = cells;
+    j.cells = (int32_t)cells;
+    j.G = G;
+    j.mu_log = std::log(efficiency) - efficiency_std * efficiency_std / 2.0;   // capture.jl:30
+    j.sigma_log = efficiency_std;
+    j.k0 = (uint32_t)seed;
+    j.k1 = (uint32_t)(seed >> 32);
+    j.net_id = net_id;
+    hipError_t e = hipSuccess;
+    for (int g = 0; g < G && e == hipSuccess; ++g)
+        e = hipMemcpy((double*)dX + (size_t)g * cells, X + (size_t)g * ld, cells * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dJ, &j, sizeof(j), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_capture_f64(1, (int64_t)GC, (CaptureJob*)dJ, (double*)dX, (int32_t*)dY, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    for (int g = 0; g < G && e == hipSuccess; ++g)
+        e = hipMemcpy(Y + (size_t)g * ld, (int32_t*)dY + (size_t)g * cells, cells * sizeof(int32_t), hipMemcpyDeviceToHost);
+    cleanup();
+    if (e != hipSuccess) return set_error(SCRNA_ERR_DEVICE, "capture failed: %s", hipGetErrorString(e));
+    return SCRNA_OK;
+}
+
+}  // extern "C"

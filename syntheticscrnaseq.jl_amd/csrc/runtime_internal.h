@@ -1,0 +1,97 @@
+// Internal types shared by the libscrna host runtime translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/scrna.h"
+#include "blob.h"
+#include "cle_kernels.h"
+
+struct scrna_ctx {
+    int device = 0;
+    uint64_t budget = 0;
+    std::atomic<uint64_t> used{0};
+    size_t lds_max = 65536;          // per-workgroup LDS limit of the device
+    size_t lds_limit_soft = 65536;   // keep double buffers under this to fit 2+ groups per CU
+    int num_cu = 0;
+    int force_ch = 0;                // tile override (autotuner / tests); 0 = automatic
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+};
+
+namespace scrna {
+
+constexpr int kMaxGenes = 256;
+constexpr uint64_t kSnapPoolBit = 1ull << 63;
+
+struct CompiledNet {
+    std::vector<uint32_t> blob;
+    size_t blob_off = 0;
+    int G = 0, S = 0, CH = 2, NW = 1, n_edges = 0;
+    int64_t cells = 0, n_steps = 0, snap_every = 0, n_snap = 0;
+    int readout = 1, precision = 0;
+    bool capture = false, population = false;
+    scrna_capture cap{};
+    scrna_population pop{};
+    uint64_t seed = 0;
+    uint32_t net_id = 0;
+    size_t off_state = 0, off_snap = 0, off_prom = 0, off_vol = 0, off_inj = 0, off_counts = 0, off_mom = 0;
+    int n_mom = 1;
+};
+
+struct LaunchGroup {
+    int CH = 2, NW = 1, tile0 = 0, n_tiles = 0, nbuf = 1;
+    size_t lds = 0;
+};
+
+}  // namespace scrna
+
+struct scrna_plan {
+    scrna_ctx* ctx = nullptr;
+    std::vector<scrna::CompiledNet> nets;
+    std::vector<scrna::LaunchGroup> groups;
+    std::vector<std::pair<void*, size_t>> allocs;
+    uint64_t bytes = 0;
+    double cell_gene_steps = 0.0;
+    bool launched = false, has_population = false;
+    uint32_t* d_blobs = nullptr;
+    scrna::Tile* d_tiles = nullptr;
+    float* d_state = nullptr;
+    float* d_snap = nullptr;
+    uint8_t* d_prom = nullptr;
+    float* d_vol = nullptr;
+    float* d_inj = nullptr;
+    int32_t* d_counts = nullptr;
+    double* d_mom = nullptr;
+    scrna::CaptureJob* d_cjobs = nullptr;
+    scrna::MomentJob* d_mjobs_snap = nullptr;
+    scrna::MomentJob* d_mjobs_state = nullptr;
+    std::vector<scrna::MomentJob> mjobs_snap, mjobs_state;
+    int n_cjobs = 0;
+    int64_t max_cap_elems = 0;
+    int max_mom_G = 0;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+};
+
+namespace scrna {
+int set_error(int code, const char* fmt, ...);
+int64_t steps_for(double T, double dt);
+int ctx_alloc(scrna_ctx* ctx, void** p, size_t bytes);
+void ctx_free(scrna_ctx* ctx, void* p, size_t bytes);
+int validate_inputs(const scrna_network* net, const scrna_kinetics* kin, const scrna_run* run,
+                    const scrna_population* pop, const scrna_capture* cap);
+int compile_network(const scrna_network* net, const scrna_kinetics* kin, const scrna_run* run,
+                    const scrna_population* pop, int force_ch, CompiledNet* out);
+int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna_kinetics* kins,
+               const scrna_run* runs, const scrna_population* pops, const scrna_capture* caps, int force_ch,
+               scrna_plan** out);
+int plan_launch(scrna_plan* P, hipStream_t st);
+int plan_download(scrna_plan* P, int32_t i, scrna_output* out);
+void plan_release(scrna_plan* P);
+int population_launch(scrna_plan* P, hipStream_t st);
+}  // namespace scrna
