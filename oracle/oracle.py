@@ -176,12 +176,17 @@ def cle(net, kin, *, dt, T=None, n_steps=None, cell_num, cell_lo=0, cell_hi=None
     return dict(m=m, p=p, snaps_m=sm[:n_snap], snaps_p=sp[:n_snap], promoter=pr)
 
 
-def cle_population(net, kin, pop, *, dt, T, cell_num, seed=0, net_id=0, init="steady"):
+def cle_population(net, kin, pop, *, dt, T, cell_num, seed=0, net_id=0, init="steady", injected=None):
     lib = load()
     n_steps = int(np.ceil(T / dt))
     s, keep = _net(net)
+    inj_p = None
+    if injected is not None:
+        inj = np.ascontiguousarray(injected, dtype=np.float32)
+        keep.append(inj)
+        inj_p = inj.ctypes.data_as(_fp)
     run = OrcRun(dt, n_steps, cell_num, 0, cell_num, seed & 0xFFFFFFFFFFFFFFFF, net_id,
-                 0 if init == "steady" else 1, 0, None, 1)
+                 0 if init == "steady" else 1, 0, inj_p, 1)
     part = {"halve": 0, "binomial": 1}[pop.partition]
     rep = {"uniform": 0, "exclude_mother": 1}[pop.replace]
     op = OrcPopulation(1, pop.growth_rate, pop.V_div, pop.V_init[0], pop.V_init[1], pop.div_check_interval,

@@ -546,9 +546,15 @@ int orc_cle_population(const orc_network* net, const orc_kinetics* kin, const or
             uint32_t u[4];
             for (int g = 0; g < G; ++g) {
                 double xm, xp;
-                if ((g & 1) == 0) orc_draw(run->seed, (uint32_t)c, (uint32_t)s, (uint32_t)(g >> 1), c3base | ORC_P_NORMAL, u);
-                int o = (g & 1) * 2;
-                orc_normal_pair(u[o], u[o + 1], &xm, &xp);
+                if (run->injected) {
+                    size_t base = ((size_t)s * G + g) * 2;
+                    xm = run->injected[(base + 0) * (size_t)N + c];
+                    xp = run->injected[(base + 1) * (size_t)N + c];
+                } else {
+                    if ((g & 1) == 0) orc_draw(run->seed, (uint32_t)c, (uint32_t)s, (uint32_t)(g >> 1), c3base | ORC_P_NORMAL, u);
+                    int o = (g & 1) * 2;
+                    orc_normal_pair(u[o], u[o + 1], &xm, &xp);
+                }
                 double tx = oc[g] ? (reg[g] + net->basals[g]) * V[c] : 0.0;   /* cle.jl:224-225 */
                 double mm = Mc[g];
                 double var_m = tx + mu_m * (mm > 0.0 ? mm : 0.0);

@@ -29,8 +29,8 @@ enum BlobHeader : int {
     H_INVK,           // f32 1/K_d
     H_HILLN,          // f32 Hill coefficient
     H_SQRTDT,         // f32 sqrt(dt) (injected-noise scaling)
-    H_GROWTH,         // f32 exp(growth_rate*dt) (1 without population)
-    H_VDIV,           // f32 division volume
+    H_POPMODE,        // bit0: binomial partition (else halve), bit1: replacement excludes the mother
+    H_DIVINT,         // steps between division checks
     H_OFF_BETA,       // [G] f32 basal rates
     H_OFF_SLOT,       // [G] i32 source slot of each gene, -1 if none
     H_OFF_INOFF,      // [G+1] i32 CSR offsets (edge index) of additive in-edges per target
@@ -51,6 +51,14 @@ enum BlobHeader : int {
     H_INJ_HI,
     H_CH,             // genes per lane the network was compiled for
     H_NW,             // waves per workgroup (NW*CH >= G)
+    H_GROWTH_LO,      // f64 exp(growth_rate*dt) (1 without population)
+    H_GROWTH_HI,
+    H_VDIV_LO,        // f64 division volume
+    H_VDIV_HI,
+    H_GP,             // dense path: G rounded up to a multiple of 32
+    H_OFF_BIAS,       // dense path: [G] f32 rowsum(A_neg)
+    H_DENSE_LO,       // dense path: offset (bf16x8 units) of the packed W^T in the weight pool
+    H_DENSE_HI,
     H_WORDS = 48
 };
 
@@ -74,12 +82,17 @@ enum TeleWord : int {
 
 enum GateType : uint32_t { GATE_AND = 0u, GATE_OR = 1u };
 
-// Tile of work: one workgroup = up to 64 cells of one network.
+// Tile of work: one workgroup = up to 64 cells (16 on the dense path) of one
+// network, advanced over steps [s0, s1) intersected with the launch's range
+// and the network's horizon.  A tile whose range starts after step 0 resumes
+// from the state pool.
 struct Tile {
     int32_t blob;        // word offset of the network's blob
     int32_t cell0;       // first cell
-    int32_t ncell;       // cells in this tile (<= 64)
+    int32_t ncell;       // cells in this tile
     int32_t net;         // network index inside the plan
+    int32_t s0, s1;      // step range of this tile
+    int32_t pad0, pad1;
 };
 
 }  // namespace scrna

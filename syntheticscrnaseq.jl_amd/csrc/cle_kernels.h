@@ -28,13 +28,28 @@ struct MomentJob {
 size_t stepper_lds_bytes(int S, int nbuf);
 hipError_t set_stepper_lds_limit(size_t bytes);
 hipError_t launch_stepper(int CH, int NW, int n_tiles, const uint32_t* blobs, const Tile* tiles,
-                          float* state_pool, float* snap_pool, uint8_t* prom_pool, float* vol_pool,
-                          const float* inj_pool, int s_begin, int s_end, int resume, int nbuf,
+                          float* state_pool, float* snap_pool, uint8_t* prom_pool, double* vol_pool,
+                          const float* inj_pool, int s_begin, int s_end, int nbuf,
                           size_t lds_bytes, hipStream_t stream);
 hipError_t launch_capture_f32(int n_jobs, int64_t max_elems, const CaptureJob* jobs, const float* src,
                               int32_t* dst, hipStream_t stream);
 hipError_t launch_capture_f64(int n_jobs, int64_t max_elems, const CaptureJob* jobs, const double* src,
                               int32_t* dst, hipStream_t stream);
+hipError_t launch_pop_init(int n_tiles, const uint32_t* blobs, const Tile* tiles, double* vol, const double* vinit,
+                           hipStream_t stream);
+hipError_t launch_pop_divide(int n_tiles, const uint32_t* blobs, const Tile* tiles, float* state, uint8_t* prom,
+                             double* vol, float* dstate, uint8_t* dprom, double* dvol, int32_t* claim, int s_div,
+                             hipStream_t stream);
+hipError_t launch_pop_place(int n_tiles, const uint32_t* blobs, const Tile* tiles, float* state, uint8_t* prom,
+                            double* vol, const float* dstate, const uint8_t* dprom, const double* dvol, int32_t* claim,
+                            hipStream_t stream);
+size_t dense_lds_bytes(int Gp, int S, int mode);
+hipError_t set_dense_lds_limit(size_t bytes);
+int dense_nb(int mode);
+hipError_t launch_dense(int mode, bool tele, bool inject, int NW, int n_tiles, const uint32_t* blobs,
+                        const Tile* tiles, const void* wpool, float* state_pool, float* snap_pool, uint8_t* prom_pool,
+                        const float* inj_pool, int s_begin, int s_end, size_t lds_bytes,
+                        hipStream_t stream);
 hipError_t launch_moments(int n_jobs, int max_G, const MomentJob* jobs, const float* pool, double* out,
                           hipStream_t stream);
 

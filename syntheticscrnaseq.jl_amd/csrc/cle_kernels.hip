@@ -52,13 +52,15 @@ template <int CH>
 __global__ void __launch_bounds__(1024)
 cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ tiles,
                    float* __restrict__ state_pool, float* __restrict__ snap_pool,
-                   uint8_t* __restrict__ prom_pool, float* __restrict__ vol_pool,
-                   const float* __restrict__ inj_pool, int s_begin, int s_end, int resume,
+                   uint8_t* __restrict__ prom_pool, double* __restrict__ vol_pool,
+                   const float* __restrict__ inj_pool, int s_begin, int s_end,
                    int nbuf) {
     extern __shared__ float lds[];
     const Tile tile = tiles[blockIdx.x];
     const uint32_t* __restrict__ B = blobs + tile.blob;
     const int G = (int)B[H_G];
+    const int s_start = max(s_begin, tile.s0);
+    const bool resume = s_start > 0;       // later chunks continue from the state pool
     const int S = (int)B[H_S];
     const uint32_t flags = B[H_FLAGS];
     const int n_steps = (int)B[H_NSTEPS];
@@ -75,7 +77,7 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
     const float invK = as_f(B[H_INVK]);
     const float hill_n = as_f(B[H_HILLN]);
     const float sqrt_dt = as_f(B[H_SQRTDT]);
-    const float growth = as_f(B[H_GROWTH]);
+    const double growth = __longlong_as_double((long long)u64_at(B, H_GROWTH_LO));
     const bool hill2 = (flags & F_HILL2) != 0;
     const bool tele = (flags & F_TELE) != 0;
     const bool inject = (flags & F_INJECT) != 0;
@@ -102,7 +104,7 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
 
     float m[CH], p[CH];
     uint32_t on = 0;                                       // promoter bit k = gene g0+k
-    float V = 1.0f;
+    double V = 1.0;                                       // cell volume (population runs)
     bool any_switch = false;
 #pragma unroll
     for (int k = 0; k < CH; ++k) {
@@ -134,8 +136,8 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
     any_switch = tele && any_switch;
 
     const int rows = 2 * S;
-    const int s_stop = min(s_end, n_steps);               // networks of a batch may differ in horizon
-    for (int s = s_begin; s < s_stop; ++s) {
+    const int s_stop = min(min(s_end, tile.s1), n_steps);               // networks of a batch may differ in horizon
+    for (int s = s_start; s < s_stop; ++s) {
         float* __restrict__ H = lds + (nbuf == 2 ? (s & 1) * rows * 64 : 0);
 
         // Telegraph switching first (tauleap_binomial.jl:110-121).
@@ -224,7 +226,7 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
                         const bool o = (on >> (k + j)) & 1u;
                         // telegraph gating of the transcription propensity
                         // (tauleap_binomial.jl:134-137); volume scaling (cle.jl:224-225)
-                        const float tx = o ? (reg + beta) * V : 0.0f;
+                        const float tx = o ? (reg + beta) * (float)V : 0.0f;
                         // mRNA (cle.jl:110-113)
                         const float mm = m[k + j];
                         const float var_m = fmaxf(fmaf(mu_m, fmaxf(mm, 0.0f), tx), 0.0f);
@@ -338,32 +340,168 @@ moments_kernel(const MomentJob* __restrict__ jobs, const float* __restrict__ poo
 }
 
 // ---------------------------------------------------------------------------
+// Population dynamics (Moran process), one thread per cell of a tile.
+//
+// pop_init_kernel: V ~ U(V_init_lo, V_init_hi) (cle.jl:158-159, population.jl:364-365).
+__global__ void __launch_bounds__(64)
+pop_init_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ tiles, double* __restrict__ vol_pool,
+                const double* __restrict__ vinit /* [2*n_nets] lo, hi */) {
+#pragma clang fp contract(off)
+    const Tile tile = tiles[blockIdx.x];
+    if ((int)threadIdx.x >= tile.ncell) return;
+    const uint32_t* __restrict__ B = blobs + tile.blob;
+    const int cell = tile.cell0 + threadIdx.x;
+    const U4 u = philox4x32_10((uint32_t)cell, 0u, 0u, (B[H_NETID] << 4) | kVInit, B[H_SEED_LO], B[H_SEED_HI]);
+    const double lo = vinit[2 * tile.net], hi = vinit[2 * tile.net + 1];
+    vol_pool[(size_t)u64_at(B, H_OUT_V_LO) + cell] = lo + (hi - lo) * u52(u.x, u.y);
+}
+
+// Division phase 1 (population.jl:414-444 / cle.jl:241-261): every cell with
+// V > V_div after step s_div splits its molecules with its daughter -- binomially
+// (round half-even, then Binomial(n, 1/2)) or by exact halving -- halves its
+// volume, buffers the daughter, and claims a uniformly drawn replacement slot.
+// Later dividers (higher cell index) win a contested slot, which is what the
+// reference's sequential phase-2 loop produces.
+__global__ void __launch_bounds__(64)
+pop_divide_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ tiles,
+                  float* __restrict__ state_pool, uint8_t* __restrict__ prom_pool, double* __restrict__ vol_pool,
+                  float* __restrict__ dstate, uint8_t* __restrict__ dprom, double* __restrict__ dvol,
+                  int32_t* __restrict__ claim, int s_div) {
+#pragma clang fp contract(off)
+    const Tile tile = tiles[blockIdx.x];
+    if ((int)threadIdx.x >= tile.ncell) return;
+    const uint32_t* __restrict__ B = blobs + tile.blob;
+    if (s_div >= (int)B[H_NSTEPS]) return;
+    const int G = (int)B[H_G];
+    const int C = (int)B[H_CELLS];
+    const int cell = tile.cell0 + threadIdx.x;
+    const size_t off_state = (size_t)u64_at(B, H_OUT_FINAL_LO);
+    const size_t off_prom = (size_t)u64_at(B, H_OUT_PROM_LO);
+    const size_t off_vol = (size_t)u64_at(B, H_OUT_V_LO);
+    const double vdiv = __longlong_as_double((long long)u64_at(B, H_VDIV_LO));
+    double V = vol_pool[off_vol + cell];
+    if (!(V > vdiv)) return;
+    const uint32_t mode = B[H_POPMODE];
+    const uint32_t k0 = B[H_SEED_LO], k1 = B[H_SEED_HI];
+    const uint32_t net = B[H_NETID];
+    for (int g = 0; g < G; ++g) {
+        for (int sp = 0; sp < 2; ++sp) {
+            const size_t i = off_state + (size_t)(sp * G + g) * C + cell;
+            const float x = state_pool[i];
+            float mother, daughter;
+            if (mode & 1u) {
+                int64_t n = (int64_t)rint((double)x);
+                if (n < 0) n = 0;
+                int64_t d = 0;
+                if (n > 0) {
+                    UStream st((uint32_t)cell, (uint32_t)s_div, ((uint32_t)(g * 2 + sp)) << 16,
+                               (net << 4) | kPartition, k0, k1);
+                    d = binomial_sample(n, 0.5, st);
+                }
+                mother = (float)(n - d);
+                daughter = (float)d;
+            } else {
+                mother = daughter = x / 2.0f;
+            }
+            state_pool[i] = mother;
+            dstate[i] = daughter;
+        }
+        dprom[off_prom + (size_t)g * C + cell] = prom_pool[off_prom + (size_t)g * C + cell];
+    }
+    V /= 2.0;
+    vol_pool[off_vol + cell] = V;
+    dvol[off_vol + cell] = V;
+    const U4 u = philox4x32_10((uint32_t)cell, (uint32_t)s_div, 0u, (net << 4) | kReplace, k0, k1);
+    const uint64_t x = ((uint64_t)u.x << 32) | (uint64_t)u.y;
+    int64_t r;
+    if ((mode & 2u) && C > 1) {                        // cle.jl:265-268
+        r = (int64_t)__umul64hi(x, (uint64_t)(C - 1));
+        if (r >= cell) r += 1;
+    } else {                                           // population.jl:448
+        r = (int64_t)__umul64hi(x, (uint64_t)C);
+    }
+    atomicMax(&claim[off_vol + r], cell + 1);
+}
+
+// Division phase 2: each claimed slot takes its winning daughter.
+__global__ void __launch_bounds__(64)
+pop_place_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ tiles,
+                 float* __restrict__ state_pool, uint8_t* __restrict__ prom_pool, double* __restrict__ vol_pool,
+                 const float* __restrict__ dstate, const uint8_t* __restrict__ dprom,
+                 const double* __restrict__ dvol, int32_t* __restrict__ claim) {
+    const Tile tile = tiles[blockIdx.x];
+    if ((int)threadIdx.x >= tile.ncell) return;
+    const uint32_t* __restrict__ B = blobs + tile.blob;
+    const int G = (int)B[H_G];
+    const int C = (int)B[H_CELLS];
+    const int r = tile.cell0 + threadIdx.x;
+    const size_t off_state = (size_t)u64_at(B, H_OUT_FINAL_LO);
+    const size_t off_prom = (size_t)u64_at(B, H_OUT_PROM_LO);
+    const size_t off_vol = (size_t)u64_at(B, H_OUT_V_LO);
+    const int w = claim[off_vol + r];
+    if (w == 0) return;
+    const int mother = w - 1;
+    for (int g = 0; g < G; ++g) {
+        state_pool[off_state + (size_t)g * C + r] = dstate[off_state + (size_t)g * C + mother];
+        state_pool[off_state + (size_t)(G + g) * C + r] = dstate[off_state + (size_t)(G + g) * C + mother];
+        prom_pool[off_prom + (size_t)g * C + r] = dprom[off_prom + (size_t)g * C + mother];
+    }
+    vol_pool[off_vol + r] = dvol[off_vol + mother];
+    claim[off_vol + r] = 0;
+}
+
+// ---------------------------------------------------------------------------
 // Host-side launchers.
+
+hipError_t launch_pop_init(int n_tiles, const uint32_t* blobs, const Tile* tiles, double* vol, const double* vinit,
+                           hipStream_t stream) {
+    if (n_tiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(pop_init_kernel, dim3(n_tiles), dim3(64), 0, stream, blobs, tiles, vol, vinit);
+    return hipGetLastError();
+}
+
+hipError_t launch_pop_divide(int n_tiles, const uint32_t* blobs, const Tile* tiles, float* state, uint8_t* prom,
+                             double* vol, float* dstate, uint8_t* dprom, double* dvol, int32_t* claim, int s_div,
+                             hipStream_t stream) {
+    if (n_tiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(pop_divide_kernel, dim3(n_tiles), dim3(64), 0, stream, blobs, tiles, state, prom, vol, dstate,
+                       dprom, dvol, claim, s_div);
+    return hipGetLastError();
+}
+
+hipError_t launch_pop_place(int n_tiles, const uint32_t* blobs, const Tile* tiles, float* state, uint8_t* prom,
+                            double* vol, const float* dstate, const uint8_t* dprom, const double* dvol, int32_t* claim,
+                            hipStream_t stream) {
+    if (n_tiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(pop_place_kernel, dim3(n_tiles), dim3(64), 0, stream, blobs, tiles, state, prom, vol, dstate,
+                       dprom, dvol, claim);
+    return hipGetLastError();
+}
 
 size_t stepper_lds_bytes(int S, int nbuf) { return (size_t)nbuf * 2 * (size_t)S * 64 * sizeof(float); }
 
 hipError_t launch_stepper(int CH, int NW, int n_tiles, const uint32_t* blobs, const Tile* tiles,
-                          float* state_pool, float* snap_pool, uint8_t* prom_pool, float* vol_pool,
-                          const float* inj_pool, int s_begin, int s_end, int resume, int nbuf,
+                          float* state_pool, float* snap_pool, uint8_t* prom_pool, double* vol_pool,
+                          const float* inj_pool, int s_begin, int s_end, int nbuf,
                           size_t lds_bytes, hipStream_t stream) {
     if (n_tiles <= 0) return hipSuccess;
     const dim3 grid(n_tiles), block(64 * NW);
     switch (CH) {
         case 2:
             hipLaunchKernelGGL(cle_stepper_kernel<2>, grid, block, lds_bytes, stream, blobs, tiles, state_pool,
-                               snap_pool, prom_pool, vol_pool, inj_pool, s_begin, s_end, resume, nbuf);
+                               snap_pool, prom_pool, vol_pool, inj_pool, s_begin, s_end, nbuf);
             break;
         case 4:
             hipLaunchKernelGGL(cle_stepper_kernel<4>, grid, block, lds_bytes, stream, blobs, tiles, state_pool,
-                               snap_pool, prom_pool, vol_pool, inj_pool, s_begin, s_end, resume, nbuf);
+                               snap_pool, prom_pool, vol_pool, inj_pool, s_begin, s_end, nbuf);
             break;
         case 8:
             hipLaunchKernelGGL(cle_stepper_kernel<8>, grid, block, lds_bytes, stream, blobs, tiles, state_pool,
-                               snap_pool, prom_pool, vol_pool, inj_pool, s_begin, s_end, resume, nbuf);
+                               snap_pool, prom_pool, vol_pool, inj_pool, s_begin, s_end, nbuf);
             break;
         case 16:
             hipLaunchKernelGGL(cle_stepper_kernel<16>, grid, block, lds_bytes, stream, blobs, tiles, state_pool,
-                               snap_pool, prom_pool, vol_pool, inj_pool, s_begin, s_end, resume, nbuf);
+                               snap_pool, prom_pool, vol_pool, inj_pool, s_begin, s_end, nbuf);
             break;
         default:
             return hipErrorInvalidValue;

@@ -1,0 +1,385 @@
+// Dense-regulation CLE stepper for large networks, additive regulation on
+// BF16 MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulate).
+//
+// Regulation (cle.jl:79-86) is A_pos*act + A_neg*rep.  With rep = 1 - act it
+// equals W*act + b, W = A_pos - A_neg (the signed A_eff of network.jl:206-219)
+// and b = rowsum(A_neg): one signed contraction per step,
+//     reg[cells x G] = act[cells x G] * W^T[G x G] + b.
+// MODE 1 (BF16): act and W rounded to bf16.
+// MODE 2 (BF16X3): act = a_hi + a_lo, W = W_hi + W_lo (bf16 each) and
+//   reg = a_hi W_hi + a_lo W_hi + a_hi W_lo, recovering ~16 mantissa bits.
+// AND / OR gates stay in FP32 on the VALU from an FP32 copy of their
+// sources' Hill values.
+//
+// Work mapping.  A workgroup owns 16 cells and NW = Gp/(16*NB) waves (Gp = G
+// rounded up to 32); wave w owns NB blocks of 16 output genes, block b being
+// genes 16*(w*NB+b) + [0,16).  In a 16x16 MFMA result lane l holds gene
+// 16*(w*NB+b) + (l & 15) for the 4 cells 4*(l>>4) + v, v = 0..3, so those
+// cells' (m, p) stay in the lane's VGPRs for the whole horizon, in
+// accumulator layout.  W^T for the wave's output genes is register-resident
+// as the B operand (4 VGPRs per 32-wide K chunk, gene block and bf16 plane).
+// NB = 2 halves the waves per workgroup, which gives each wave the 256-VGPR
+// budget of 2 waves per SIMD that BF16X3's two weight planes need.  Each step
+// every lane writes act of its genes and cells into an LDS tile laid out
+// [cell][gene] -- the A operand's layout -- the workgroup synchronises once
+// (double-buffered tile), and each wave issues NB*Gp/32 MFMAs (x3 for BF16X3).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "blob.h"
+#include "cle_kernels.h"
+#include "device_rng.h"
+
+namespace scrna {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float d_as_f(uint32_t w) { return __uint_as_float(w); }
+__device__ __forceinline__ uint64_t d_u64(const uint32_t* B, int i) {
+    return (uint64_t)B[i] | ((uint64_t)B[i + 1] << 32);
+}
+// Word (i & 3) of a Philox block without a dynamically indexed array.
+__device__ __forceinline__ uint32_t pick4(const U4& u, int i) {
+    return (i & 2) ? ((i & 1) ? u.w : u.z) : ((i & 1) ? u.y : u.x);
+}
+
+constexpr int kDenseMaxKC = 8;   // Gp <= 256
+
+template <int MODE, int NB, bool TELE, bool INJECT>
+__global__ void __launch_bounds__(1024 / NB)
+cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ tiles,
+                 const bf16x8* __restrict__ wpool, float* __restrict__ state_pool, float* __restrict__ snap_pool,
+                 uint8_t* __restrict__ prom_pool, const float* __restrict__ inj_pool, int s_begin, int s_end) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char dlds[];
+    const Tile tile = tiles[blockIdx.x];
+    const uint32_t* __restrict__ B = blobs + tile.blob;
+    const int G = (int)B[H_G];
+    const int s_start = max(s_begin, tile.s0);
+    const bool resume = s_start > 0;       // later chunks continue from the state pool
+    const int S = (int)B[H_S];             // gate-source slots
+    const int Gp = (int)B[H_GP];           // G rounded up to 32
+    const int KC = Gp / 32;
+    const uint32_t flags = B[H_FLAGS];
+    const int n_steps = (int)B[H_NSTEPS];
+    const int snap_every = (int)B[H_SNAP_EVERY];
+    const int C = (int)B[H_CELLS];
+    const uint32_t k0 = B[H_SEED_LO], k1 = B[H_SEED_HI];
+    const uint32_t c3n = (B[H_NETID] << 4) | kNormal;
+    const uint32_t c3s = (B[H_NETID] << 4) | kSwitch;
+    const float dt = d_as_f(B[H_DT]);
+    const float neg2ln2dt = d_as_f(B[H_NEG2LN2DT]);
+    const float k_t = d_as_f(B[H_KT]);
+    const float mu_m = d_as_f(B[H_MUM]);
+    const float mu_p = d_as_f(B[H_MUP]);
+    const float invK = d_as_f(B[H_INVK]);
+    const float hill_n = d_as_f(B[H_HILLN]);
+    const float sqrt_dt = d_as_f(B[H_SQRTDT]);
+    const bool hill2 = (flags & F_HILL2) != 0;
+    const bool tele = TELE;                 // specialised: telegraph and injected-noise paths cost registers
+    const bool inject = INJECT;
+    const size_t off_state = (size_t)d_u64(B, H_OUT_FINAL_LO);
+    const size_t off_snap = (size_t)d_u64(B, H_OUT_SNAP_LO);
+    const size_t off_prom = (size_t)d_u64(B, H_OUT_PROM_LO);
+    const size_t off_inj = (size_t)d_u64(B, H_INJ_LO);
+    const size_t off_w = (size_t)d_u64(B, H_DENSE_LO);   // in bf16x8 units
+    const int NBLK = Gp / 16;              // gene blocks in the network
+
+    const int l = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int rg = l >> 4;                 // row group: cells 4*rg .. 4*rg+3 of the tile
+    const int par = l & 1;
+
+    // LDS: act tiles [plane][buffer][16 rows][Gp + 8] bf16, then FP32 gate Hill values.
+    const int rowstride = Gp + 8;
+    const int tile_elems = 16 * rowstride;
+    __bf16* act_t = (__bf16*)dlds;
+    float* ghill = (float*)(dlds + (size_t)(MODE == 2 ? 4 : 2) * tile_elems * sizeof(__bf16));
+
+    const int32_t* __restrict__ gate_off = (const int32_t*)(B + B[H_OFF_GATEOFF]);
+    const uint32_t* __restrict__ gates = B + B[H_OFF_GATES];
+    const uint32_t* __restrict__ telew = B + B[H_OFF_TELE];
+
+    // Per gene block: weights, constants, state.
+    bf16x8 whi[NB][kDenseMaxKC], wlo[NB][kDenseMaxKC];
+    int g[NB], slot[NB], q0[NB], q1[NB];
+    bool gl[NB];
+    float beta[NB], bias[NB];
+    uint32_t tflags[NB];
+    uint64_t thr_on[NB], thr_off[NB];
+    float m[NB][4], p[NB][4];
+    uint32_t on = 0;                       // promoter bit 4*b + v
+
+    auto row = [&](int v) { return 4 * rg + v; };
+    auto live = [&](int v) { return row(v) < tile.ncell; };
+    auto cell = [&](int v) { return tile.cell0 + (live(v) ? row(v) : 0); };
+
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int blk = w * NB + b;
+        g[b] = blk * 16 + (l & 15);
+        gl[b] = g[b] < G;
+        const int gc = gl[b] ? g[b] : 0;
+#pragma unroll
+        for (int kc = 0; kc < kDenseMaxKC; ++kc) {
+            if (kc < KC) {
+                whi[b][kc] = wpool[off_w + ((size_t)blk * KC + kc) * 64 + l];
+                if (MODE == 2) wlo[b][kc] = wpool[off_w + ((size_t)NBLK * KC + (size_t)blk * KC + kc) * 64 + l];
+            }
+        }
+        beta[b] = d_as_f(B[B[H_OFF_BETA] + gc]);
+        bias[b] = d_as_f(B[B[H_OFF_BIAS] + gc]);
+        slot[b] = gl[b] ? ((const int32_t*)(B + B[H_OFF_SLOT]))[g[b]] : -1;
+        tflags[b] = gl[b] ? telew[g[b] * 8 + T_FLAGS] : 0u;
+        thr_on[b] = d_u64(telew, gc * 8 + T_ON_LO);
+        thr_off[b] = d_u64(telew, gc * 8 + T_OFF_LO);
+        q0[b] = gl[b] ? gate_off[g[b]] : 0;
+        q1[b] = gl[b] ? gate_off[g[b] + 1] : 0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            m[b][v] = 0.0f;
+            p[b][v] = 0.0f;
+            if (gl[b]) {
+                if (resume) {
+                    m[b][v] = state_pool[off_state + (size_t)g[b] * C + cell(v)];
+                    p[b][v] = state_pool[off_state + (size_t)(G + g[b]) * C + cell(v)];
+                    on |= (uint32_t)(prom_pool[off_prom + (size_t)g[b] * C + cell(v)] != 0) << (4 * b + v);
+                } else {
+                    const uint32_t* initw = B + B[H_OFF_INIT];
+                    m[b][v] = d_as_f(initw[2 * g[b]]);
+                    p[b][v] = d_as_f(initw[2 * g[b] + 1]);
+                    bool o = true;
+                    if (tele && (tflags[b] & 2u)) {
+                        const U4 u = philox4x32_10((uint32_t)cell(v), 0u, (uint32_t)(g[b] >> 2),
+                                                   (B[H_NETID] << 4) | kPromInit, k0, k1);
+                        o = (uint64_t)pick4(u, g[b]) < d_u64(telew, g[b] * 8 + T_INIT_LO);
+                    }
+                    on |= (uint32_t)o << (4 * b + v);
+                }
+            }
+        }
+    }
+
+    const int s_stop = min(min(s_end, tile.s1), n_steps);
+    for (int s = s_start; s < s_stop; ++s) {
+        const int buf = s & 1;
+        __bf16* T0 = act_t + (size_t)buf * tile_elems;
+        __bf16* T1 = act_t + (size_t)(2 + buf) * tile_elems;    // low plane (BF16X3)
+        float* GH = ghill + (size_t)buf * 2 * S * 16;
+
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            // Telegraph switching first (tauleap_binomial.jl:110-121).  The switch
+            // block (cell, s, g>>2) serves the 4 genes of a lane quad; lane j of
+            // the quad draws the block of its element v = j and the quad rotates
+            // words so every lane gets its gene's word for each of its cells.
+            if (tele) {
+                const U4 u = philox4x32_10((uint32_t)cell(l & 3), (uint32_t)s, (uint32_t)(g[b] >> 2), c3s, k0, k1);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int j = l & 3;
+                    const int src = (l & ~3) | ((j - r) & 3);      // drew element (j - r) & 3
+                    const uint32_t send = pick4(u, (j + r) & 3);    // word of receiver (j + r) & 3
+                    const uint32_t got = (uint32_t)__shfl((int)send, src);
+                    const int bit = 4 * b + ((j - r) & 3);
+                    if (tflags[b] & 1u) {
+                        const bool cur = (on >> bit) & 1u;
+                        const bool flip = (uint64_t)got < (cur ? thr_off[b] : thr_on[b]);
+                        on ^= (uint32_t)flip << bit;
+                    }
+                }
+            }
+            // Hill values of the lane's gene for its 4 cells -> LDS.
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                float act = 0.0f, rep = 1.0f;
+                if (gl[b]) {
+                    const float x = p[b][v] * invK;
+                    float xn = hill2 ? x * x : __builtin_amdgcn_exp2f(hill_n * __builtin_amdgcn_logf(x));
+                    xn = fminf(xn, 1e30f);
+                    rep = __builtin_amdgcn_rcpf(1.0f + xn);
+                    act = xn * rep;
+                }
+                const __bf16 hi = (__bf16)act;
+                T0[row(v) * rowstride + g[b]] = hi;
+                if (MODE == 2) T1[row(v) * rowstride + g[b]] = (__bf16)(act - (float)hi);
+                if (slot[b] >= 0) {
+                    GH[(2 * slot[b]) * 16 + row(v)] = act;
+                    GH[(2 * slot[b] + 1) * 16 + row(v)] = rep;
+                }
+            }
+        }
+        __syncthreads();
+
+        // reg = act * W^T on the matrix cores.
+        f32x4 acc[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) acc[b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        const __bf16* arow0 = T0 + (l & 15) * rowstride + 8 * rg;
+        const __bf16* arow1 = T1 + (l & 15) * rowstride + 8 * rg;
+#pragma unroll
+        for (int kc = 0; kc < kDenseMaxKC; ++kc) {
+            if (kc < KC) {
+                const bf16x8 a = *(const bf16x8*)(arow0 + 32 * kc);
+                bf16x8 alo;
+                if (MODE == 2) alo = *(const bf16x8*)(arow1 + 32 * kc);
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, whi[b][kc], acc[b], 0, 0, 0);
+                    if (MODE == 2) {
+                        acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, whi[b][kc], acc[b], 0, 0, 0);
+                        acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wlo[b][kc], acc[b], 0, 0, 0);
+                    }
+                }
+            }
+        }
+
+        // Elementwise update of element (b, v) (cle.jl:108-120).
+        auto update = [&](const int b, const int v, float xm, float xp) {
+            float reg = acc[b][v] + bias[b];
+            for (int q = q0[b]; q < q1[b];) {                   // AND / OR (cle.jl:89-106)
+                const uint32_t hd = gates[q];
+                const uint32_t type = hd >> 24;
+                const int ns = (int)(hd & 0xffffu);
+                float prod = 1.0f;
+                for (int i = 0; i < ns; ++i) prod *= GH[(2 * (int)gates[q + 2 + i] + (int)type) * 16 + row(v)];
+                reg = fmaf(d_as_f(gates[q + 1]), type == GATE_AND ? prod : 1.0f - prod, reg);
+                q += 2 + ns;
+            }
+            const bool o = (on >> (4 * b + v)) & 1u;
+            const float tx = o ? reg + beta[b] : 0.0f;
+            const float mm = m[b][v];
+            const float var_m = fmaxf(fmaf(mu_m, fmaxf(mm, 0.0f), tx), 0.0f);
+            float mn = fmaf(fmaf(-mu_m, mm, tx), dt, mm);
+            mn = fmaxf(fmaf(xm, __builtin_amdgcn_sqrtf(var_m), mn), 0.0f);
+            const float pp = p[b][v];
+            const float var_p = fmaxf(fmaf(k_t, mn, mu_p * fmaxf(pp, 0.0f)), 0.0f);
+            float pn = fmaf(fmaf(k_t, mn, -mu_p * pp), dt, pp);
+            pn = fmaxf(fmaf(xp, __builtin_amdgcn_sqrtf(var_p), pn), 0.0f);
+            m[b][v] = gl[b] ? mn : 0.0f;
+            p[b][v] = gl[b] ? pn : 0.0f;
+        };
+
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            if (inject) {
+                const size_t base = ((size_t)s * G + (gl[b] ? g[b] : 0)) * 2;
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    update(b, v, inj_pool[off_inj + base * C + cell(v)] * sqrt_dt,
+                           inj_pool[off_inj + (base + 1) * C + cell(v)] * sqrt_dt);
+            } else {
+                // The normal block (cell, s, g>>1) carries genes g (words 0,1) and
+                // g^1 (words 2,3), i.e. lanes l and l^1, which own the same cells.
+                // The even lane draws element t's block, the odd lane element
+                // t+2's, and each hands its partner the partner's half.
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    const int cm = par ? cell(t + 2) : cell(t);
+                    const U4 u = philox4x32_10((uint32_t)cm, (uint32_t)s, (uint32_t)(g[b] >> 1), c3n, k0, k1);
+                    const uint32_t mine_a = par ? u.z : u.x, mine_b = par ? u.w : u.y;
+                    const uint32_t got_a = (uint32_t)__shfl_xor((int)(par ? u.x : u.z), 1);
+                    const uint32_t got_b = (uint32_t)__shfl_xor((int)(par ? u.y : u.w), 1);
+                    float xm, xp;
+                    normal_pair_scaled(par ? got_a : mine_a, par ? got_b : mine_b, neg2ln2dt, xm, xp);
+                    update(b, t, xm, xp);
+                    normal_pair_scaled(par ? mine_a : got_a, par ? mine_b : got_b, neg2ln2dt, xm, xp);
+                    update(b, t + 2, xm, xp);
+                }
+            }
+        }
+
+        if (snap_every > 0 && ((s + 1) % snap_every) == 0) {
+            const int snap = (s + 1) / snap_every - 1;
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                if (!gl[b]) continue;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    if (live(v)) {
+                        snap_pool[off_snap + ((size_t)(2 * snap) * G + g[b]) * C + cell(v)] = m[b][v];
+                        snap_pool[off_snap + ((size_t)(2 * snap + 1) * G + g[b]) * C + cell(v)] = p[b][v];
+                    }
+                }
+            }
+        }
+    }
+
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        if (!gl[b]) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            if (live(v)) {
+                state_pool[off_state + (size_t)g[b] * C + cell(v)] = m[b][v];
+                state_pool[off_state + (size_t)(G + g[b]) * C + cell(v)] = p[b][v];
+                prom_pool[off_prom + (size_t)g[b] * C + cell(v)] = (uint8_t)((on >> (4 * b + v)) & 1u);
+            }
+        }
+    }
+}
+
+// Gene blocks per wave for a precision mode (BF16X3 needs the larger budget).
+int dense_nb(int mode) { return mode == 2 ? 2 : 1; }
+
+size_t dense_lds_bytes(int Gp, int S, int mode) {
+    const size_t tile = (size_t)16 * (Gp + 8) * 2;
+    return (size_t)(mode == 2 ? 4 : 2) * tile + (size_t)2 * 2 * S * 16 * sizeof(float);
+}
+
+template <int MODE, int NB>
+static hipError_t dense_attr(size_t bytes) {
+    const void* fns[] = {(const void*)cle_dense_kernel<MODE, NB, false, false>,
+                         (const void*)cle_dense_kernel<MODE, NB, false, true>,
+                         (const void*)cle_dense_kernel<MODE, NB, true, false>,
+                         (const void*)cle_dense_kernel<MODE, NB, true, true>};
+    hipError_t e = hipSuccess;
+    for (const void* f : fns) {
+        hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (r != hipSuccess) e = r;
+    }
+    return e;
+}
+
+hipError_t set_dense_lds_limit(size_t bytes) {
+    hipError_t e1 = dense_attr<1, 1>(bytes);
+    hipError_t e2 = dense_attr<2, 2>(bytes);
+    return e1 != hipSuccess ? e1 : e2;
+}
+
+template <int MODE, int NB>
+static void dense_go(bool tele, bool inject, dim3 grid, dim3 block, size_t lds, hipStream_t st, const uint32_t* blobs,
+                     const Tile* tiles, const bf16x8* wpool, float* state, float* snap, uint8_t* prom,
+                     const float* inj, int s_begin, int s_end) {
+    if (tele && inject)
+        hipLaunchKernelGGL((cle_dense_kernel<MODE, NB, true, true>), grid, block, lds, st, blobs, tiles, wpool, state,
+                           snap, prom, inj, s_begin, s_end);
+    else if (tele)
+        hipLaunchKernelGGL((cle_dense_kernel<MODE, NB, true, false>), grid, block, lds, st, blobs, tiles, wpool, state,
+                           snap, prom, inj, s_begin, s_end);
+    else if (inject)
+        hipLaunchKernelGGL((cle_dense_kernel<MODE, NB, false, true>), grid, block, lds, st, blobs, tiles, wpool, state,
+                           snap, prom, inj, s_begin, s_end);
+    else
+        hipLaunchKernelGGL((cle_dense_kernel<MODE, NB, false, false>), grid, block, lds, st, blobs, tiles, wpool,
+                           state, snap, prom, inj, s_begin, s_end);
+}
+
+hipError_t launch_dense(int mode, bool tele, bool inject, int NW, int n_tiles, const uint32_t* blobs,
+                        const Tile* tiles, const void* wpool, float* state_pool, float* snap_pool, uint8_t* prom_pool,
+                        const float* inj_pool, int s_begin, int s_end, size_t lds_bytes,
+                        hipStream_t stream) {
+    if (n_tiles <= 0) return hipSuccess;
+    const dim3 grid(n_tiles), block(64 * NW);
+    if (mode == 1)
+        dense_go<1, 1>(tele, inject, grid, block, lds_bytes, stream, blobs, tiles, (const bf16x8*)wpool, state_pool,
+                       snap_pool, prom_pool, inj_pool, s_begin, s_end);
+    else if (mode == 2)
+        dense_go<2, 2>(tele, inject, grid, block, lds_bytes, stream, blobs, tiles, (const bf16x8*)wpool, state_pool,
+                       snap_pool, prom_pool, inj_pool, s_begin, s_end);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+}  // namespace scrna

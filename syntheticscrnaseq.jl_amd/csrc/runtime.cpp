@@ -57,6 +57,20 @@ static inline uint32_t f2w(float f) {
     return w;
 }
 
+// float -> bfloat16 bits, round to nearest even (finite inputs).
+static inline uint16_t f2bf(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+static inline float bf2f(uint16_t h) {
+    const uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
 // u = u32 * 2^-32 in [0,1); "u < q" <=> u32 < ceil(q * 2^32).
 static uint64_t prob_threshold(double q) {
     if (!(q > 0.0)) return 0;
@@ -202,15 +216,32 @@ int compile_network(const scrna_network* net, const scrna_kinetics* kin, const s
     }
     out->CH = CH;
     out->NW = NW;
+    // Dense MFMA path: the additive regulation becomes act * W^T + b.
+    const bool dense = run->precision != SCRNA_PREC_FP32;
+    out->dense = dense;
+    out->mode = run->precision;
+    out->Gp = (G + 31) / 32 * 32;
+    std::vector<double> Wd, bias;
+    if (dense) {
+        Wd.assign((size_t)G * G, 0.0);
+        bias.assign(G, 0.0);
+        out->NW = out->Gp / (16 * dense_nb(out->mode));
+    }
 
-    // A_eff split (network.jl:206-219), as per-target CSR over source genes.
+    // A_eff split (network.jl:206-219), as per-target CSR over source genes
+    // (sparse path) or as the signed dense W = A_pos - A_neg plus
+    // b = rowsum(A_neg) (dense path, using rep = 1 - act).
     std::vector<int> is_src(G, 0);
     std::vector<std::vector<std::pair<int, double>>> in_edges(G);
     for (int i = 0; i < G; ++i)
         for (int j = 0; j < G; ++j) {
             if (i == j) continue;
             const double aeff = net->A[i + (size_t)j * G] * net->basals[i];
-            if (aeff != 0.0) {
+            if (aeff == 0.0) continue;
+            if (dense) {
+                Wd[(size_t)i * G + j] = aeff;
+                if (aeff < 0.0) bias[i] += -aeff;
+            } else {
                 in_edges[i].push_back({j, aeff});
                 is_src[j] = 1;
             }
@@ -280,8 +311,19 @@ int compile_network(const scrna_network* net, const scrna_kinetics* kin, const s
     b[H_INVK] = f2w((float)(1.0 / kin->K_d));
     b[H_HILLN] = f2w((float)kin->n);
     b[H_SQRTDT] = f2w((float)std::sqrt(run->dt));
-    b[H_GROWTH] = f2w(population ? (float)std::exp(pop->growth_rate * run->dt) : 1.0f);
-    b[H_VDIV] = f2w(population ? (float)pop->V_div : 0.0f);
+    b[H_POPMODE] = population ? (uint32_t)((pop->partition == 1 ? 1u : 0u) | (pop->replace_mode == 1 ? 2u : 0u)) : 0u;
+    b[H_DIVINT] = population ? (uint32_t)pop->div_check_interval : 0u;
+    {
+        const double growth = population ? std::exp(pop->growth_rate * run->dt) : 1.0;   // cle.jl:169
+        const double vdiv = population ? pop->V_div : 0.0;
+        uint64_t gb, vb;
+        std::memcpy(&gb, &growth, 8);
+        std::memcpy(&vb, &vdiv, 8);
+        b[H_GROWTH_LO] = (uint32_t)gb;
+        b[H_GROWTH_HI] = (uint32_t)(gb >> 32);
+        b[H_VDIV_LO] = (uint32_t)vb;
+        b[H_VDIV_HI] = (uint32_t)(vb >> 32);
+    }
     b[H_CH] = (uint32_t)CH;
     b[H_NW] = (uint32_t)NW;
 
@@ -364,6 +406,28 @@ int compile_network(const scrna_network* net, const scrna_kinetics* kin, const s
         b.push_back(f2w(p0));
     }
     align16();
+    b[H_GP] = (uint32_t)out->Gp;
+    b[H_OFF_BIAS] = (uint32_t)b.size();
+    for (int g = 0; g < G; ++g) b.push_back(f2w(dense ? (float)bias[g] : 0.0f));
+    align16();
+    if (dense) {
+        // W^T as MFMA B operands: [plane][gene block][K chunk][lane][8] bf16,
+        // lane l of block blk / chunk kc holding W[16 blk + (l&15)][32 kc + 8 (l>>4) + e].
+        const int Gp = out->Gp, NBLK = Gp / 16, KC = Gp / 32;
+        const int planes = out->mode == SCRNA_PREC_BF16X3 ? 2 : 1;
+        out->wpack.assign((size_t)planes * NBLK * KC * 64 * 8, 0);
+        for (int blk = 0; blk < NBLK; ++blk)
+            for (int kc = 0; kc < KC; ++kc)
+                for (int l = 0; l < 64; ++l)
+                    for (int e = 0; e < 8; ++e) {
+                        const int gi = 16 * blk + (l & 15), gj = 32 * kc + 8 * (l >> 4) + e;
+                        const float wv = (gi < G && gj < G) ? (float)Wd[(size_t)gi * G + gj] : 0.0f;
+                        const size_t idx = (((size_t)blk * KC + kc) * 64 + l) * 8 + e;
+                        const uint16_t hi = f2bf(wv);
+                        out->wpack[idx] = hi;
+                        if (planes == 2) out->wpack[(size_t)NBLK * KC * 64 * 8 + idx] = f2bf(wv - bf2f(hi));
+                    }
+    }
     return SCRNA_OK;
 }
 
@@ -401,8 +465,6 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
     for (int i = 0; i < n; ++i) {
         int rc = validate_inputs(&nets[i], &kins[i], &runs[i], pops ? &pops[i] : nullptr, caps ? &caps[i] : nullptr);
         if (rc) return set_error(rc, "network %d: %s", i, g_last_error.c_str());
-        if (runs[i].precision != SCRNA_PREC_FP32 && !(nets[i].G >= 16))
-            return set_error(SCRNA_ERR_INVALID, "network %d: MFMA precision modes need G >= 16", i);
     }
     std::unique_ptr<scrna_plan> P(new scrna_plan());
     P->ctx = ctx;
@@ -445,6 +507,8 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
         blobs.insert(blobs.end(), cn.blob.begin(), cn.blob.end());
         if (blobs.size() > (size_t)INT32_MAX) return set_error(SCRNA_ERR_INVALID, "network descriptors too large");
         if (cn.population) P->has_population = true;
+        if (cn.population && cn.dense)
+            return set_error(SCRNA_ERR_UNSUPPORTED, "network %d: population dynamics run with FP32 regulation only", i);
     }
     if (P->has_population) {
         for (int i = 0; i < n; ++i)
@@ -452,34 +516,64 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
                 return set_error(SCRNA_ERR_UNSUPPORTED, "a plan mixes population and non-population networks");
     }
 
-    // Launch groups: networks sharing (CH, NW); tiles of 64 cells.
-    std::map<std::pair<int, int>, std::vector<Tile>> groups;
-    std::map<std::pair<int, int>, int> group_S;
+    // Launch groups: sparse networks sharing (CH, NW) in tiles of 64 cells;
+    // dense networks sharing (mode, NW, telegraph, injected noise) in tiles of 16.
+    struct GKey {
+        int kind, a, NW, tele, inj;
+        bool operator<(const GKey& o) const {
+            return std::tie(kind, a, NW, tele, inj) < std::tie(o.kind, o.a, o.NW, o.tele, o.inj);
+        }
+    };
+    std::map<GKey, std::vector<Tile>> groups;
+    std::map<GKey, std::pair<int, int>> group_SG;   // max S, max Gp
+    size_t w_elems = 0;
     for (int i = 0; i < n; ++i) {
-        const CompiledNet& cn = P->nets[i];
-        auto key = std::make_pair(cn.CH, cn.NW);
+        CompiledNet& cn = P->nets[i];
+        const bool tele = (cn.blob[H_FLAGS] & F_TELE) != 0, inj = (cn.blob[H_FLAGS] & F_INJECT) != 0;
+        GKey key = cn.dense ? GKey{1, cn.mode, cn.NW, tele ? 1 : 0, inj ? 1 : 0} : GKey{0, cn.CH, cn.NW, 0, 0};
         auto& v = groups[key];
-        int& s = group_S[key];
-        s = std::max(s, cn.S);
-        for (int64_t c0 = 0; c0 < cn.cells; c0 += 64) {
+        auto& sg = group_SG[key];
+        sg.first = std::max(sg.first, cn.S);
+        sg.second = std::max(sg.second, cn.Gp);
+        const int per = cn.dense ? 16 : 64;
+        for (int64_t c0 = 0; c0 < cn.cells; c0 += per) {
             Tile t;
             t.blob = (int32_t)cn.blob_off;
             t.cell0 = (int32_t)c0;
-            t.ncell = (int32_t)std::min<int64_t>(64, cn.cells - c0);
+            t.ncell = (int32_t)std::min<int64_t>(per, cn.cells - c0);
             t.net = i;
+            t.s0 = 0;
+            t.s1 = INT32_MAX;
+            t.pad0 = t.pad1 = 0;
             v.push_back(t);
+        }
+        if (cn.dense) {
+            cn.off_w = w_elems / 8;
+            w_elems += cn.wpack.size();
+            uint32_t* hb = blobs.data() + cn.blob_off;
+            hb[H_DENSE_LO] = (uint32_t)cn.off_w;
+            hb[H_DENSE_HI] = (uint32_t)((uint64_t)cn.off_w >> 32);
         }
     }
     std::vector<Tile> all_tiles;
     for (auto& kv : groups) {
         LaunchGroup lg;
-        lg.CH = kv.first.first;
-        lg.NW = kv.first.second;
+        lg.kind = kv.first.kind;
+        lg.NW = kv.first.NW;
         lg.tile0 = (int)all_tiles.size();
         lg.n_tiles = (int)kv.second.size();
-        const int S = std::max(1, group_S[kv.first]);
-        lg.nbuf = stepper_lds_bytes(S, 2) <= ctx->lds_limit_soft ? 2 : 1;
-        lg.lds = stepper_lds_bytes(S, lg.nbuf);
+        const int S = std::max(1, group_SG[kv.first].first);
+        if (lg.kind == 0) {
+            lg.CH = kv.first.a;
+            lg.nbuf = stepper_lds_bytes(S, 2) <= ctx->lds_limit_soft ? 2 : 1;
+            lg.lds = stepper_lds_bytes(S, lg.nbuf);
+        } else {
+            lg.mode = kv.first.a;
+            lg.tele = kv.first.tele != 0;
+            lg.inject = kv.first.inj != 0;
+            lg.nbuf = 2;
+            lg.lds = dense_lds_bytes(group_SG[kv.first].second, S, lg.mode);
+        }
         if (lg.lds > ctx->lds_max)
             return set_error(SCRNA_ERR_UNSUPPORTED, "%d source genes need %zu B of LDS per workgroup (max %zu)", S,
                              lg.lds, ctx->lds_max);
@@ -555,15 +649,20 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
     PLAN_ALLOC(d_state, float, state_elems);
     PLAN_ALLOC(d_snap, float, snap_elems);
     PLAN_ALLOC(d_prom, uint8_t, prom_elems);
-    PLAN_ALLOC(d_vol, float, vol_elems);
+    PLAN_ALLOC(d_vol, double, vol_elems);
     PLAN_ALLOC(d_inj, float, inj_elems);
     PLAN_ALLOC(d_counts, int32_t, count_elems);
     PLAN_ALLOC(d_mom, double, mom_elems);
     PLAN_ALLOC(d_cjobs, CaptureJob, cjobs.size());
     PLAN_ALLOC(d_mjobs_snap, MomentJob, P->mjobs_snap.size());
     PLAN_ALLOC(d_mjobs_state, MomentJob, P->mjobs_state.size());
+    PLAN_ALLOC(d_wpool, uint16_t, w_elems);
 #undef PLAN_ALLOC
     P->n_cjobs = (int)cjobs.size();
+    if (P->has_population && (rc = population_prepare(P.get(), state_elems, prom_elems, vol_elems))) {
+        plan_release(P.get());
+        return rc;
+    }
     hipStream_t st = ctx->stream;
     auto up = [&](void* dst, const void* src, size_t bytes) -> int {
         if (bytes == 0) return SCRNA_OK;
@@ -578,6 +677,13 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
         (rc = up(P->d_mjobs_state, P->mjobs_state.data(), P->mjobs_state.size() * sizeof(MomentJob)))) {
         plan_release(P.get());
         return rc;
+    }
+    for (int i = 0; i < n; ++i) {
+        const CompiledNet& cn = P->nets[i];
+        if (cn.dense && (rc = up(P->d_wpool + cn.off_w * 8, cn.wpack.data(), cn.wpack.size() * sizeof(uint16_t)))) {
+            plan_release(P.get());
+            return rc;
+        }
     }
     for (int i = 0; i < n; ++i) {
         if (!runs[i].injected_noise) continue;
@@ -602,6 +708,21 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
     return SCRNA_OK;
 }
 
+// Run steps [s_begin, s_end) of every network (clamped to each horizon).
+// A tile resumes from the state pool exactly when its first step is > 0.
+int launch_steps(scrna_plan* P, hipStream_t st, int s_begin, int s_end) {
+    for (const LaunchGroup& lg : P->groups) {
+        if (lg.kind == 0)
+            HIP_TRY(launch_stepper(lg.CH, lg.NW, lg.n_tiles, P->d_blobs, P->d_tiles + lg.tile0, P->d_state,
+                                   P->d_snap, P->d_prom, P->d_vol, P->d_inj, s_begin, s_end, lg.nbuf, lg.lds, st));
+        else
+            HIP_TRY(launch_dense(lg.mode, lg.tele, lg.inject, lg.NW, lg.n_tiles, P->d_blobs, P->d_tiles + lg.tile0,
+                                 P->d_wpool, P->d_state, P->d_snap, P->d_prom, P->d_inj, s_begin, s_end, lg.lds,
+                                 st));
+    }
+    return SCRNA_OK;
+}
+
 int plan_launch(scrna_plan* P, hipStream_t st) {
     if (!st) st = P->ctx->stream;
     HIP_TRY(hipSetDevice(P->ctx->device));
@@ -610,9 +731,8 @@ int plan_launch(scrna_plan* P, hipStream_t st) {
         int rc = population_launch(P, st);
         if (rc) return rc;
     } else {
-        for (const LaunchGroup& lg : P->groups)
-            HIP_TRY(launch_stepper(lg.CH, lg.NW, lg.n_tiles, P->d_blobs, P->d_tiles + lg.tile0, P->d_state,
-                                   P->d_snap, P->d_prom, P->d_vol, P->d_inj, 0, INT32_MAX, 0, lg.nbuf, lg.lds, st));
+        int rc = launch_steps(P, st, 0, INT32_MAX);
+        if (rc) return rc;
     }
     HIP_TRY(hipEventRecord(P->ev_stop, st));
     HIP_TRY(launch_moments((int)P->mjobs_snap.size(), P->max_mom_G, P->d_mjobs_snap, P->d_snap, P->d_mom, st));
@@ -663,9 +783,7 @@ int plan_download(scrna_plan* P, int32_t i, scrna_output* out) {
         HIP_TRY(hipMemcpy(out->moments, P->d_mom + cn.off_mom, (size_t)cn.n_mom * G * 5 * sizeof(double),
                           hipMemcpyDeviceToHost));
     if (out->volumes && cn.population) {
-        std::vector<float> v(C);
-        HIP_TRY(hipMemcpy(v.data(), P->d_vol + cn.off_vol, C * sizeof(float), hipMemcpyDeviceToHost));
-        for (size_t c = 0; c < C; ++c) out->volumes[c] = (double)v[c];
+        HIP_TRY(hipMemcpy(out->volumes, P->d_vol + cn.off_vol, C * sizeof(double), hipMemcpyDeviceToHost));
     }
     return SCRNA_OK;
 }
@@ -708,7 +826,8 @@ int scrna_ctx_create(int32_t device, uint64_t hbm_budget_bytes, scrna_ctx** out)
     C->lds_max = (size_t)lds;
     C->lds_limit_soft = std::min<size_t>(C->lds_max, 64 * 1024);
     C->num_cu = prop.multiProcessorCount;
-    if (C->lds_max > 65536 && set_stepper_lds_limit(C->lds_max) != hipSuccess) {
+    if (C->lds_max > 65536 && (set_stepper_lds_limit(C->lds_max) != hipSuccess ||
+                               set_dense_lds_limit(C->lds_max) != hipSuccess)) {
         (void)hipGetLastError();
         C->lds_max = 65536;   // large dynamic LDS not granted: stay within the default limit
     }

@@ -42,10 +42,18 @@ struct CompiledNet {
     uint32_t net_id = 0;
     size_t off_state = 0, off_snap = 0, off_prom = 0, off_vol = 0, off_inj = 0, off_counts = 0, off_mom = 0;
     int n_mom = 1;
+    // dense (MFMA) regulation path
+    bool dense = false;
+    int mode = 0, Gp = 0;
+    std::vector<uint16_t> wpack;     // bf16 bits
+    size_t off_w = 0;                // in 8-element (16-byte) units
 };
 
 struct LaunchGroup {
+    int kind = 0;                    // 0: sparse VALU stepper, 1: dense MFMA stepper
     int CH = 2, NW = 1, tile0 = 0, n_tiles = 0, nbuf = 1;
+    int mode = 0;                    // dense precision mode
+    bool tele = false, inject = false;
     size_t lds = 0;
 };
 
@@ -64,9 +72,10 @@ struct scrna_plan {
     float* d_state = nullptr;
     float* d_snap = nullptr;
     uint8_t* d_prom = nullptr;
-    float* d_vol = nullptr;
+    double* d_vol = nullptr;
     float* d_inj = nullptr;
     int32_t* d_counts = nullptr;
+    uint16_t* d_wpool = nullptr;
     double* d_mom = nullptr;
     scrna::CaptureJob* d_cjobs = nullptr;
     scrna::MomentJob* d_mjobs_snap = nullptr;
@@ -76,6 +85,14 @@ struct scrna_plan {
     int64_t max_cap_elems = 0;
     int max_mom_G = 0;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    // population buffers: daughters, claims, initial volume ranges
+    float* d_dstate = nullptr;
+    uint8_t* d_dprom = nullptr;
+    double* d_dvol = nullptr;
+    int32_t* d_claim = nullptr;
+    double* d_vinit = nullptr;
+    int pop_interval = 0;
+    int64_t pop_max_steps = 0;
 };
 
 namespace scrna {
@@ -94,4 +111,7 @@ int plan_launch(scrna_plan* P, hipStream_t st);
 int plan_download(scrna_plan* P, int32_t i, scrna_output* out);
 void plan_release(scrna_plan* P);
 int population_launch(scrna_plan* P, hipStream_t st);
+int launch_steps(scrna_plan* P, hipStream_t st, int s_begin, int s_end);
+int population_prepare(scrna_plan* P, size_t state_elems, size_t prom_elems, size_t vol_elems);
+int plan_alloc(scrna_plan* p, void** ptr, size_t bytes);
 }  // namespace scrna

@@ -1,0 +1,72 @@
+"""Dense (MFMA) regulation path against the FP64 oracle.
+
+BF16X3 recovers ~16 mantissa bits of the contraction, BF16 keeps 8; both
+are compared with the oracle on identical noise.  The G=256 check states the
+per-gene mean and variance error over a cell subset that the oracle replays
+with the identical Philox stream (the workload's acceptance band is 1%).
+"""
+import numpy as np
+import pytest
+
+import scrna_amd as sc
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+KIN = sc.KineticParams(k_t=5.0, K_d=10.0, n=2.0, mu_m=1.0, mu_p=0.2)
+
+
+def relerr(a, b):
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1.0)))
+
+
+@pytest.mark.parametrize("precision,tol", [("bf16x3", 2e-4), ("bf16", 2e-2)])
+@pytest.mark.parametrize("G,cells", [(40, 50), (96, 33)])
+def test_dense_injected_parity(ctx, precision, tol, G, cells):
+    rng = np.random.default_rng(G)
+    net = sc.samplers.mixed_network(G, rng, n_tf=(6, 8), targets_per_tf=(5, 12), n_and=(2, 3), n_or=(2, 3),
+                                    a_range=(0.5, 3.0))
+    dt, T = 0.005, 1.0
+    steps = sc.n_steps(T, dt)
+    xi = rng.standard_normal((steps, G, 2, cells)).astype(np.float32)
+    print(f"dense {precision} G={G} cells={cells}", flush=True)
+    y = sc.simulate(net, sc.CLE(dt), KIN, cell_num=cells, T=T, readout="both", injected_noise=xi, seed=3,
+                    precision=precision)
+    ref = orc.cle(net, KIN, dt=dt, n_steps=steps, cell_num=cells, injected=xi, seed=3)
+    err = relerr(y, np.concatenate([ref["m"].T, ref["p"].T], axis=1))
+    print(f"max rel err {err:.3e}")
+    assert err <= tol
+
+
+def test_dense_telegraph_matches_sparse_fp32(ctx):
+    """Telegraph promoters in the dense kernel draw the same switch stream as the
+    sparse FP32 kernel: promoter states agree exactly, states to the BF16X3 tolerance."""
+    rng = np.random.default_rng(3)
+    net = sc.samplers.mixed_network(48, rng, n_tf=(4, 5), targets_per_tf=(3, 6), n_and=(1, 2), n_or=(1, 2),
+                                    telegraph=True)
+    cells, dt, T = 100, 0.01, 2.0
+    steps = sc.n_steps(T, dt)
+    xi = rng.standard_normal((steps, net.G, 2, cells)).astype(np.float32)
+    a = sc.simulate(net, sc.CLE(dt), KIN, cell_num=cells, T=T, readout="both", injected_noise=xi, seed=8,
+                    precision="bf16x3")
+    ref = orc.cle(net, KIN, dt=dt, n_steps=steps, cell_num=cells, injected=xi, seed=8)
+    assert relerr(a, np.concatenate([ref["m"].T, ref["p"].T], axis=1)) <= 2e-4
+
+
+@pytest.mark.parametrize("precision,mean_tol,var_tol", [("bf16x3", 1e-3, 2e-3), ("bf16", 1e-2, 1e-2),
+                                                        ("fp32", 1e-3, 2e-3)])
+def test_g256_precision_moments(ctx, precision, mean_tol, var_tol):
+    net = sc.samplers.dense_g256_network()
+    cells, dt, steps, seed = 384, 0.005, 300, 17
+    T = steps * dt
+    res = sc.simulate(net, sc.CLE(dt), KIN, cell_num=cells, T=T, readout="both", seed=seed, precision=precision,
+                      full=True)
+    ref = orc.cle(net, KIN, dt=dt, n_steps=steps, cell_num=cells, seed=seed, threads=16)
+    G = net.G
+    for k, want in ((0, ref["m"]), (1, ref["p"])):
+        got = res.y[:, k * G:(k + 1) * G].T
+        mean_err = np.abs(got.mean(1) - want.mean(1)) / np.maximum(np.abs(want.mean(1)), 1e-3)
+        var_err = np.abs(got.var(1) - want.var(1)) / np.maximum(want.var(1), 1e-3)
+        print(f"{precision} species {k}: max mean err {mean_err.max():.2e}, max var err {var_err.max():.2e}")
+        assert mean_err.max() <= mean_tol
+        assert var_err.max() <= var_tol

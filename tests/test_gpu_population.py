@@ -1,0 +1,76 @@
+"""Population CLE on the GPU against the FP64 oracle (cle.jl:143-284 with the
+division rules of population.jl:157-205).
+
+Volumes are double precision on both sides and grow by the same host-computed
+factor, so division decisions and replacement slots agree exactly; the
+molecule state then agrees to the FP32 tolerance (halving) or, for binomial
+partitioning, for all cells whose rounding at division did not straddle a
+half-integer.
+"""
+import numpy as np
+import pytest
+
+import scrna_amd as sc
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+KIN = sc.KineticParams(k_t=2.0, K_d=10.0, n=2.0, mu_m=0.5, mu_p=0.2)
+
+
+def net3():
+    return sc.GeneNetwork(3, np.array([4.0, 2.0, 3.0]),
+                          np.array([[0, 0, 2.0], [-1.5, 0, 0], [0, 2.5, 0]]),
+                          [], [sc.RedunEdge(0, [1, 2], 2.0)], k_on=[0.5, np.inf, 1.0], k_off=[1.0, np.inf, 2.0])
+
+
+@pytest.mark.parametrize("partition,replace", [("halve", "exclude_mother"), ("halve", "uniform"),
+                                               ("binomial", "uniform")])
+def test_population_matches_oracle(ctx, partition, replace):
+    net = net3()
+    cells, dt, T = 300, 0.05, 15.0
+    pop = sc.PopulationConfig(cell_num=cells, growth_rate=0.1, V_div=2.0, V_init=(0.8, 1.9), div_check_interval=5,
+                              partition=partition, replace=replace)
+    steps = sc.n_steps(T, dt)
+    xi = np.random.default_rng(4).standard_normal((steps, net.G, 2, cells)).astype(np.float32)
+    print(f"population {partition}/{replace} cells={cells} steps={steps}", flush=True)
+    res = sc.simulate(net, sc.CLE(dt), KIN, population=pop, T=T, readout="both", seed=9, injected_noise=xi,
+                      full=True)
+    ref = orc.cle_population(net, KIN, pop, dt=dt, T=T, cell_num=cells, seed=9, injected=xi)
+    np.testing.assert_array_equal(res.volumes, ref["V"])
+    want = np.concatenate([ref["m"].T, ref["p"].T], axis=1)
+    err = np.abs(res.y - want) / np.maximum(np.abs(want), 1.0)
+    per_cell = err.max(axis=1)
+    if partition == "halve":
+        assert per_cell.max() <= 1e-4
+    else:
+        assert np.mean(per_cell <= 1e-4) >= 0.95
+    assert res.y.shape == (cells, 2 * net.G)
+
+
+def test_population_volume_law(ctx):
+    """Moran volume distribution p(v) = V_div/v^2 on [V_div/2, V_div]
+    (test_analytical_moments.jl:366-421): <V> = V_div ln 2, <1/V> = 3/(2 V_div),
+    median 2 V_div/3, all within 15%; N stays fixed."""
+    net = sc.GeneNetwork(1, np.array([2.0]), np.zeros((1, 1)))
+    kin = sc.KineticParams(k_t=1.0, K_d=50.0, n=2.0, mu_m=0.1, mu_p=0.2)
+    pop = sc.PopulationConfig(cell_num=5000, growth_rate=0.03, V_div=2.0, V_init=(1.0, 2.0), div_check_interval=1)
+    res = sc.simulate(net, sc.CLE(0.1), kin, population=pop, T=2000.0, readout="both", seed=42, full=True)
+    V = res.volumes
+    assert V.size == 5000 and res.y.shape == (5000, 2)
+    assert V.min() >= 0.99 and V.max() <= 2.0 * np.exp(0.03 * 0.1) * 1.01
+    rel = lambda a, b: abs(a - b) / b
+    assert rel(V.mean(), 2.0 * np.log(2.0)) < 0.15
+    assert rel(np.mean(1.0 / V), 3.0 / 4.0) < 0.15
+    assert rel(np.median(V), 4.0 / 3.0) < 0.15
+    assert np.corrcoef(res.y[:, 0], V)[0, 1] > 0.1           # larger cells transcribe more
+
+
+def test_population_reference_smoke(ctx):
+    """test_gpu_parity.jl:77-86: shape, finite, positive mean."""
+    net = sc.GeneNetwork(1, np.array([5.0]), np.zeros((1, 1)))
+    kin = sc.KineticParams(k_t=2.0, K_d=50.0, n=2.0, mu_m=0.1, mu_p=0.2)
+    pop = sc.PopulationConfig(cell_num=500, growth_rate=0.03, V_div=2.0, V_init=(0.8, 1.2), div_check_interval=5,
+                              partition="halve", replace="uniform")
+    y = sc.simulate(net, sc.CLE(0.1), kin, population=pop, T=100.0, readout="protein", seed=1)
+    assert y.shape == (500, 1) and np.all(np.isfinite(y)) and y.mean() > 0
