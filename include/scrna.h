@@ -188,7 +188,24 @@ int scrna_plan_download(scrna_plan* plan, int32_t i, scrna_output* out);
 int scrna_plan_stepper_ms(scrna_plan* plan, double* ms, double* cell_gene_steps);
 /* Device bytes held by the plan. */
 int scrna_plan_bytes(scrna_plan* plan, uint64_t* bytes);
+/* Capture the plan's whole kernel sequence (all steps, division phases,
+ * moments, capture) into one hipGraph; later scrna_plan_launch calls replay
+ * it with a single hipGraphLaunch.  Replaces the reference's host-driven
+ * per-step launch loop (ext:87-108, ext:184-205, ext:450-520). */
+int scrna_plan_use_graph(scrna_plan* plan);
 int scrna_plan_destroy(scrna_plan* plan);
+
+/* ---- tile-shape autotuner ------------------------------------------- */
+
+/* Time every legal genes-per-lane chunk (2, 4, 8, 16) of the FP32 stepper on
+ * a synthetic network per G bucket up to max_G and keep the fastest per
+ * bucket in the context; plans created afterwards use it.  With cache_path
+ * non-NULL the table is read from that JSON file when it matches the device,
+ * otherwise measured and written there.  No reference counterpart (the
+ * reference GPU path has no tunable tiles). */
+int scrna_autotune(scrna_ctx* ctx, int32_t max_G, const char* cache_path, int32_t* n_buckets);
+/* Pin the chunk for networks with G in [G_lo, G_hi]; genes_per_lane 0 clears. */
+int scrna_ctx_set_tile(scrna_ctx* ctx, int32_t G_lo, int32_t G_hi, int32_t genes_per_lane);
 
 /* ---- serving: continuous batching of simulate() requests ------------ */
 

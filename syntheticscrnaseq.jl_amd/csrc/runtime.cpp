@@ -447,6 +447,10 @@ void plan_release(scrna_plan* p) {
     if (p->ev_start) (void)hipEventDestroy(p->ev_start);
     if (p->ev_stop) (void)hipEventDestroy(p->ev_stop);
     p->ev_start = p->ev_stop = nullptr;
+    if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->graph_exec = nullptr;
+    p->graph = nullptr;
 }
 
 int plan_alloc(scrna_plan* p, void** ptr, size_t bytes) {
@@ -474,7 +478,8 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
            mom_elems = 0;
     for (int i = 0; i < n; ++i) {
         CompiledNet& cn = P->nets[i];
-        int rc = compile_network(&nets[i], &kins[i], &runs[i], pops ? &pops[i] : nullptr, force_ch, &cn);
+        const int ch = force_ch ? force_ch : ctx_tile_for(ctx, nets[i].G);   // autotuner table
+        int rc = compile_network(&nets[i], &kins[i], &runs[i], pops ? &pops[i] : nullptr, ch, &cn);
         if (rc) return rc;
         cn.readout = runs[i].readout;
         cn.precision = runs[i].precision;
@@ -723,10 +728,9 @@ int launch_steps(scrna_plan* P, hipStream_t st, int s_begin, int s_end) {
     return SCRNA_OK;
 }
 
-int plan_launch(scrna_plan* P, hipStream_t st) {
-    if (!st) st = P->ctx->stream;
-    HIP_TRY(hipSetDevice(P->ctx->device));
-    HIP_TRY(hipEventRecord(P->ev_start, st));
+// Every kernel of one simulation, in stream order (also the body of the graph).
+static int plan_enqueue(scrna_plan* P, hipStream_t st, bool events) {
+    if (events) HIP_TRY(hipEventRecord(P->ev_start, st));
     if (P->has_population) {
         int rc = population_launch(P, st);
         if (rc) return rc;
@@ -734,11 +738,55 @@ int plan_launch(scrna_plan* P, hipStream_t st) {
         int rc = launch_steps(P, st, 0, INT32_MAX);
         if (rc) return rc;
     }
-    HIP_TRY(hipEventRecord(P->ev_stop, st));
+    if (events) HIP_TRY(hipEventRecord(P->ev_stop, st));
     HIP_TRY(launch_moments((int)P->mjobs_snap.size(), P->max_mom_G, P->d_mjobs_snap, P->d_snap, P->d_mom, st));
     HIP_TRY(launch_moments((int)P->mjobs_state.size(), P->max_mom_G, P->d_mjobs_state, P->d_state, P->d_mom, st));
     HIP_TRY(launch_capture_f32(P->n_cjobs, P->max_cap_elems, P->d_cjobs, P->d_state, P->d_counts, st));
+    return SCRNA_OK;
+}
+
+int plan_launch(scrna_plan* P, hipStream_t st) {
+    if (!st) st = P->ctx->stream;
+    HIP_TRY(hipSetDevice(P->ctx->device));
+    if (P->graph_exec) {
+        // Graph replay: the events bracket the whole graph (stepper kernels dominate it).
+        HIP_TRY(hipEventRecord(P->ev_start, st));
+        HIP_TRY(hipGraphLaunch(P->graph_exec, st));
+        HIP_TRY(hipEventRecord(P->ev_stop, st));
+    } else {
+        int rc = plan_enqueue(P, st, true);
+        if (rc) return rc;
+    }
     P->launched = true;
+    return SCRNA_OK;
+}
+
+// Capture the plan's kernel sequence once into a hipGraph; later launches
+// replay it with a single hipGraphLaunch.  No memset nodes are captured.
+int plan_make_graph(scrna_plan* P) {
+    if (P->graph_exec) return SCRNA_OK;
+    HIP_TRY(hipSetDevice(P->ctx->device));
+    hipStream_t cs = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    int rc = SCRNA_OK;
+    if (e == hipSuccess) {
+        rc = plan_enqueue(P, cs, false);
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(cs, &g);
+        if (rc == SCRNA_OK && e2 == hipSuccess) {
+            e = hipGraphInstantiate(&P->graph_exec, g, nullptr, nullptr, 0);
+            P->graph = g;
+        } else if (rc == SCRNA_OK) {
+            e = e2;
+        }
+    }
+    (void)hipStreamDestroy(cs);
+    if (rc) return rc;
+    if (e != hipSuccess) {
+        P->graph_exec = nullptr;
+        return set_error(SCRNA_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+    }
     return SCRNA_OK;
 }
 
@@ -855,7 +903,7 @@ int scrna_plan_create(scrna_ctx* ctx, int32_t n, const scrna_network* nets, cons
     *out = nullptr;
     if (!ctx) return set_error(SCRNA_ERR_INVALID, "null context");
     std::lock_guard<std::mutex> lk(ctx->mu);
-    return plan_build(ctx, n, nets, kins, runs, pops, captures, ctx->force_ch, out);
+    return plan_build(ctx, n, nets, kins, runs, pops, captures, 0, out);
 }
 
 int scrna_plan_launch(scrna_plan* plan, void* stream) {
@@ -891,6 +939,12 @@ int scrna_plan_bytes(scrna_plan* plan, uint64_t* bytes) {
     if (!plan || !bytes) return set_error(SCRNA_ERR_INVALID, "null argument");
     *bytes = plan->bytes;
     return SCRNA_OK;
+}
+
+int scrna_plan_use_graph(scrna_plan* plan) {
+    if (!plan) return set_error(SCRNA_ERR_INVALID, "null plan");
+    std::lock_guard<std::mutex> lk(plan->ctx->mu);
+    return plan_make_graph(plan);
 }
 
 int scrna_plan_destroy(scrna_plan* plan) {

@@ -12,7 +12,14 @@
 #include "blob.h"
 #include "cle_kernels.h"
 
+// Genes-per-lane choice for networks with G in [G_lo, G_hi] (autotuner table).
+struct TileRule {
+    int G_lo, G_hi, ch;
+};
+
 struct scrna_ctx {
+    std::vector<TileRule> tiles;
+    std::mutex tile_mu;
     int device = 0;
     uint64_t budget = 0;
     std::atomic<uint64_t> used{0};
@@ -85,6 +92,8 @@ struct scrna_plan {
     int64_t max_cap_elems = 0;
     int max_mom_G = 0;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
     // population buffers: daughters, claims, initial volume ranges
     float* d_dstate = nullptr;
     uint8_t* d_dprom = nullptr;
@@ -112,6 +121,8 @@ int plan_download(scrna_plan* P, int32_t i, scrna_output* out);
 void plan_release(scrna_plan* P);
 int population_launch(scrna_plan* P, hipStream_t st);
 int launch_steps(scrna_plan* P, hipStream_t st, int s_begin, int s_end);
+int plan_make_graph(scrna_plan* P);
+int ctx_tile_for(scrna_ctx* ctx, int G);
 int population_prepare(scrna_plan* P, size_t state_elems, size_t prom_elems, size_t vol_elems);
 int plan_alloc(scrna_plan* p, void** ptr, size_t bytes);
 }  // namespace scrna

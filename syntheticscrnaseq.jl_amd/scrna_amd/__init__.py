@@ -12,6 +12,7 @@ from .cle import CLE, Plan, SimResult, apply_capture, n_steps, simulate, simulat
 from .kinetics import CaptureModel, KineticParams, PopulationConfig, steady_state
 from .network import (CoopEdge, GeneNetwork, RedunEdge, gene_network_from_theta, hill_activation,
                       hill_repression, is_bursty, n_species, precompute_hill_matrices)
+from .serving import Server
 
 __all__ = [
     "BudgetError", "Context", "ScrnaError", "UnsupportedError", "default_context", "load",

@@ -149,7 +149,10 @@ SIGNATURES = {
     "scrna_plan_download": (C.c_int, [_VP, C.c_int32, C.POINTER(ScrnaOutput)]),
     "scrna_plan_stepper_ms": (C.c_int, [_VP, c_double_p, c_double_p]),
     "scrna_plan_bytes": (C.c_int, [_VP, C.POINTER(C.c_uint64)]),
+    "scrna_plan_use_graph": (C.c_int, [_VP]),
     "scrna_plan_destroy": (C.c_int, [_VP]),
+    "scrna_autotune": (C.c_int, [_VP, C.c_int32, C.c_char_p, C.POINTER(C.c_int32)]),
+    "scrna_ctx_set_tile": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32]),
     "scrna_server_create": (C.c_int, [_VP, C.c_int32, C.POINTER(_VP)]),
     "scrna_submit": (C.c_int, [_VP, C.POINTER(ScrnaRequest), C.POINTER(C.c_int64)]),
     "scrna_wait": (C.c_int, [_VP, C.c_int64, C.c_int64, C.POINTER(ScrnaOutput)]),
@@ -218,6 +221,16 @@ class Context:
         h = C.c_void_p()
         check(self.lib.scrna_ctx_create(device, hbm_budget_bytes, C.byref(h)), "scrna_ctx_create")
         self.handle = h
+
+    def autotune(self, max_G: int = 256, cache_path: str | None = None) -> int:
+        """Pick genes-per-lane per G bucket by timing (scrna_autotune); returns buckets tuned."""
+        n = C.c_int32()
+        path = cache_path.encode() if cache_path else None
+        check(self.lib.scrna_autotune(self.handle, int(max_G), path, C.byref(n)), "scrna_autotune")
+        return n.value
+
+    def set_tile(self, G_lo: int, G_hi: int, genes_per_lane: int) -> None:
+        check(self.lib.scrna_ctx_set_tile(self.handle, G_lo, G_hi, genes_per_lane), "scrna_ctx_set_tile")
 
     def close(self) -> None:
         if self.handle:

@@ -241,6 +241,10 @@ class Plan:
     def sync(self) -> None:
         _lib.check(self.lib.scrna_plan_sync(self.handle), "scrna_plan_sync")
 
+    def use_graph(self) -> None:
+        """Capture the whole simulation into one hipGraph replayed by launch()."""
+        _lib.check(self.lib.scrna_plan_use_graph(self.handle), "scrna_plan_use_graph")
+
     def stepper_ms(self) -> tuple:
         ms, cgs = C.c_double(), C.c_double()
         _lib.check(self.lib.scrna_plan_stepper_ms(self.handle, C.byref(ms), C.byref(cgs)), "stepper_ms")
