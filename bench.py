@@ -1,21 +1,25 @@
 """Benchmark: GPU CLE throughput in cell*gene*steps/s on one MI355X.
 
-Workload (one "step" of this benchmark = one full simulate() pass): a seeded
+Default workload (one timed "step" = one full simulate() pass): a seeded
 G=20 mixed-regulation network with telegraph promoters on every gene,
 100,000 cells, 5000 Euler-Maruyama steps at dt=0.01, a snapshot every 500
 steps, FP64 moment sums per snapshot and LogNormal-Binomial capture on the
 final state.  Inputs (the compiled network) are resident in HBM before the
-timed region; outputs stay in HBM.
-
-Prints one JSON line.  Adds:
-  roofline      VALU issue roofline of the stepper kernel (the dominant kernel),
+timed region; outputs stay in HBM.  Prints one JSON line with
+  roofline      VALU issue roofline of the stepper kernel (the dominant one),
                 timed with HIP events on its launch stream;
   cpu_baseline  the FP64 CPU restatement (oracle/) timed on this host's cores
                 on a bounded sample of the same workload.
+
+Other workloads (--workload): g256 (precision comparison, 65,536 cells, 2000
+steps), batch (4096 uneven networks with population dynamics in one hipGraph),
+serve (Poisson arrivals at a fixed QPS), copy (plain device copy of known size,
+for scaling hardware byte counters).
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -32,13 +36,19 @@ import numpy as np  # noqa: E402
 # (the published 157.3 TFLOP/s FP32 peak is this x 2 (FMA) x 2 (packed)).
 VALU_PEAK_TOPS = 39.3216
 FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0
 
 
-def valu_census(net) -> float:
+def log(msg: str) -> None:
+    print(f"# {msg}", file=sys.stderr, flush=True)
+
+
+def valu_census(net, dense: bool = False) -> float:
     """Algorithmic VALU lane-ops per cell*gene*step for a network (fixed rule).
 
     E = 36 FP32 ops for the update (cle.jl:79-120, every +,-,*,/,max,sqrt,pow = 1);
-    R = 2 * additive in-edges per gene + 3 * gate source incidences per gene;
+    R = 2 * additive in-edges per gene (sparse path; 0 on the MFMA path) +
+        3 * gate source incidences per gene;
     telegraph = 3 ops; quarter-rate surcharge +3 for each of 2 sqrt and 1 rcp;
     Box-Muller for one normal pair (log, sqrt, sin, cos at 4 each + 5) = 21;
     Philox4x32-10 = 138 ops per call (20 32x32 products at 4 + 58 xor/add),
@@ -47,7 +57,7 @@ def valu_census(net) -> float:
     G = net.G
     A = np.array(net.interactions)
     np.fill_diagonal(A, 0.0)
-    d_add = np.count_nonzero(A) / G
+    d_add = 0.0 if dense else np.count_nonzero(A) / G
     inc = sum(len(e.sources) for e in list(net.cooperative) + list(net.redundant)) / G
     tele = bool(np.any(np.isfinite(net.k_on)))
     ops = 36 + 2 * d_add + 3 * inc + 9 + 21 + 138 * 0.5
@@ -63,7 +73,7 @@ def cpu_baseline(wl, target_s: float = 12.0) -> dict:
     threads = max(1, min(16, os.cpu_count() or 1))
     net, kin = wl["net"], wl["kin"]
     steps = int(np.ceil(wl["T"] / wl["dt"]))
-    probe_cells = 4 * threads
+    probe_cells = 16 * threads
     t0 = time.perf_counter()
     orc.cle(net, kin, dt=wl["dt"], n_steps=steps, cell_num=wl["cells"], cell_lo=0, cell_hi=probe_cells,
             seed=1, threads=threads)
@@ -78,25 +88,25 @@ def cpu_baseline(wl, target_s: float = 12.0) -> dict:
     cgs = float(cells) * net.G * steps
     return {"value": cgs / el, "unit": "cell*gene*steps/s", "cores": threads, "kind": "port",
             "sample": f"{cells} of {wl['cells']} cells x G={net.G} x {steps} steps (full horizon), "
-                      f"FP64 C restatement of cle.jl with telegraph + AND/OR, OpenMP over cells, "
-                      f"{el:.1f} s"}
+                      f"FP64 C restatement of cle.jl with telegraph + AND/OR, OpenMP over cells, {el:.1f} s"}
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cells", type=int, default=None, help="override cell count (testing only)")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    a = ap.parse_args()
-    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(cells and networks would shard across GPUs with no exchange; see DESIGN.md).",
-              file=sys.stderr)
-        return 2
+def timed_runs(plan, steps: int, warmup: int):
+    for _ in range(warmup):
+        plan.launch()
+    plan.sync()
+    log(f"warmup done ({warmup}), timing {steps} runs")
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.launch()
+        ms, _ = plan.stepper_ms()          # waits for this run's stepper kernels
+        kernel_ms.append(ms)
+    plan.sync()
+    return time.perf_counter() - t0, kernel_ms
 
-    import scrna_amd as sc
+
+def run_g20(a, sc) -> dict:
     from scrna_amd import samplers
 
     wl = samplers.gpu_g20_workload()
@@ -107,25 +117,10 @@ def main() -> int:
                    capture=wl["capture"], seed=20251203, init=wl["init"], snapshot_every=wl["snapshot_every"])
     steps_per_run = sc.n_steps(wl["T"], wl["dt"])
     cgs_per_run = float(net.G) * wl["cells"] * steps_per_run
-
-    for _ in range(a.warmup):
-        plan.launch()
-    plan.sync()
-    print(f"# warmup done ({a.warmup}), timing {a.steps} runs", file=sys.stderr, flush=True)
-    kernel_ms = []
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        plan.launch()
-        ms, _ = plan.stepper_ms()          # waits for this run's stepper kernel
-        kernel_ms.append(ms)
-    plan.sync()
-    el = time.perf_counter() - t0
-
+    el, kernel_ms = timed_runs(plan, a.steps, a.warmup)
     res = plan.download(0)
     if not np.all(np.isfinite(res.y)):
-        print("non-finite state", file=sys.stderr)
-        return 1
-
+        raise SystemExit("non-finite state")
     value = cgs_per_run * a.steps / el
     k_ms = float(np.mean(kernel_ms))
     census = valu_census(net)
@@ -153,11 +148,189 @@ def main() -> int:
                      "kernel": "cle_stepper_kernel", "kernel_ms": k_ms, "census_ops_per_cgs": census,
                      "fp32_equiv": {"achieved": 4 * achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s"}},
     }
+    if a.tile:
+        out["config"]["genes_per_lane"] = a.tile
     if not a.no_cpu:
-        print("# cpu baseline", file=sys.stderr, flush=True)
+        log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(wl)
     plan.close()
-    print(json.dumps(out))
+    return out
+
+
+def run_g256(a, sc) -> dict:
+    """Three contraction forms at G=256 (65,536 cells, 2000 steps, dt=0.005) and
+    their per-gene mean/variance error against an FP64 replay of a cell subset
+    with the identical Philox stream."""
+    from scrna_amd import samplers
+
+    net = samplers.dense_g256_network()
+    kin = samplers.WORKLOAD_KINETICS
+    cells = a.cells or 65536
+    dt, steps = 0.005, 2000
+    T = steps * dt
+    n_sub = a.subset
+    out = {"workload": "G=256 additive network, 25% dense A, 8 AND + 8 OR, 65536 cells, 2000 steps dt=0.005",
+           "cells": cells, "steps": steps, "results": {}}
+    subsets = {}
+    for prec in ("fp32", "bf16", "bf16x3"):
+        plan = sc.Plan([net], sc.CLE(dt), kin, cell_num=cells, T=T, readout="both", seed=777, precision=prec)
+        el, kms = timed_runs(plan, a.steps, a.warmup)
+        cgs = float(net.G) * cells * steps
+        k_ms = float(np.mean(kms))
+        census = valu_census(net, dense=prec != "fp32")
+        r = {"cgs_per_s": cgs * a.steps / el, "kernel_ms": k_ms,
+             "valu_frac": census * cgs / (k_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS,
+             "census_ops_per_cgs": census}
+        if prec != "fp32":
+            mfma_flops = 2.0 * net.G * cgs * (3 if prec == "bf16x3" else 1)
+            r["mfma_tflops_issued"] = mfma_flops / (k_ms * 1e-3) / 1e12
+        y = plan.download(0).y
+        subsets[prec] = y[:n_sub]
+        out["results"][prec] = r
+        log(f"{prec}: {r['cgs_per_s']:.3e} c*g*s/s, kernel {k_ms:.1f} ms")
+        plan.close()
+    if n_sub > 0 and not a.no_cpu:
+        from oracle import oracle as orc
+        t0 = time.perf_counter()
+        ref = orc.cle(net, kin, dt=dt, n_steps=steps, cell_num=cells, cell_lo=0, cell_hi=n_sub, seed=777,
+                      threads=max(1, min(16, os.cpu_count() or 1)))
+        log(f"FP64 replay of {n_sub} cells: {time.perf_counter() - t0:.1f} s")
+        G = net.G
+        for prec, y in subsets.items():
+            errs = {}
+            for k, want in ((0, ref["m"]), (1, ref["p"])):
+                got = y[:, k * G:(k + 1) * G].T
+                me = np.abs(got.mean(1) - want.mean(1)) / np.maximum(np.abs(want.mean(1)), 1e-3)
+                ve = np.abs(got.var(1) - want.var(1)) / np.maximum(want.var(1), 1e-3)
+                errs[("mrna", "protein")[k]] = {"max_mean_rel_err": float(me.max()), "max_var_rel_err": float(ve.max())}
+            out["results"][prec]["vs_fp64_replay"] = errs
+        out["replay_cells"] = n_sub
+    base = out["results"]["fp32"]["cgs_per_s"]
+    for prec in ("bf16", "bf16x3"):
+        out["results"][prec]["speedup_vs_fp32_fused"] = out["results"][prec]["cgs_per_s"] / base
+    return out
+
+
+def run_batch(a, sc) -> dict:
+    """4096 uneven networks with Moran population dynamics in one hipGraph."""
+    from scrna_amd import samplers
+
+    n = a.networks
+    nets, cells, pop = samplers.batch_networks(n)
+    kin = samplers.WORKLOAD_KINETICS
+    T, dt = a.T or 100.0, 0.01
+    pops = [sc.PopulationConfig(cell_num=c, growth_rate=pop.growth_rate, V_div=pop.V_div, V_init=pop.V_init,
+                                div_check_interval=pop.div_check_interval, partition=pop.partition,
+                                replace=pop.replace) for c in cells]
+    t0 = time.perf_counter()
+    plan = sc.Plan(nets, sc.CLE(dt), kin, population=pops, T=T, readout="mrna", seed=99)
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    plan.use_graph()
+    t_graph = time.perf_counter() - t0
+    el, kms = timed_runs(plan, a.steps, a.warmup)
+    steps = sc.n_steps(T, dt)
+    cgs = float(sum(net.G * c for net, c in zip(nets, cells))) * steps
+    sample = plan.download(0)
+    return {"workload": f"{n} networks, G~U{{5..64}}, cells logU[256,8192], population (growth 0.05, V_div 2, "
+                        f"binomial partition, Moran), telegraph, T={T}, dt={dt}, one hipGraph",
+            "networks": n, "total_cells": int(sum(cells)), "mean_G": float(np.mean([x.G for x in nets])),
+            "cell_gene_steps": cgs, "cgs_per_s": cgs * a.steps / el, "s_per_run": el / a.steps,
+            "graph_kernel_ms": float(np.mean(kms)), "plan_build_s": t_build, "graph_capture_s": t_graph,
+            "finite": bool(np.all(np.isfinite(sample.y)))}
+
+
+def run_serve(a, sc) -> dict:
+    """Poisson arrivals at a fixed QPS; G in {10,50,200}, cells logU[1k,50k],
+    T ~ U(10,200) at dt=0.01, half telegraph, capture on every request."""
+    from scrna_amd import samplers
+
+    rng = np.random.default_rng(5)
+    kin = samplers.WORKLOAD_KINETICS
+    qps, n_req = a.qps, a.requests
+    reqs = []
+    for i in range(n_req):
+        G = int(rng.choice([10, 50, 200]))
+        tele = bool(i % 2)
+        net = samplers.mixed_network(G, rng, n_tf=(2, 3) if G <= 10 else (4, 5),
+                                     targets_per_tf=(2, 4) if G <= 10 else (3, 6), n_and=(0, 1) if G <= 10 else (1, 2),
+                                     n_or=(0, 1) if G <= 10 else (1, 2), telegraph=tele)
+        cells = int(np.exp(rng.uniform(np.log(1000), np.log(50000))))
+        T = float(rng.uniform(10, 200))
+        reqs.append((net, cells, T))
+    gaps = rng.exponential(1.0 / qps, size=n_req)
+    cap = sc.CaptureModel(0.1, 0.3, "mrna")
+    with sc.Server(max_slots=64) as srv:
+        t_start = time.perf_counter()
+        tickets = []
+        t_next = t_start
+        for i, (net, cells, T) in enumerate(reqs):
+            t_next += gaps[i]
+            while time.perf_counter() < t_next:
+                time.sleep(min(0.01, t_next - time.perf_counter()))
+            tickets.append(srv.submit(net, sc.CLE(0.01), kin, cell_num=cells, T=T, readout="mrna", capture=cap,
+                                      seed=1000 + i))
+            if i % 10 == 9:
+                log(f"submitted {i + 1}/{n_req}")
+        for t in tickets:
+            srv.wait(t)
+        wall = time.perf_counter() - t_start
+        st = srv.stats()
+    return {"workload": f"Poisson arrivals at {qps} req/s, {n_req} requests, G in {{10,50,200}}, cells logU[1k,50k], "
+                        "T~U(10,200) dt=0.01, half telegraph, LogNormal-Binomial capture",
+            "qps": qps, "requests": n_req, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
+            "max_ms": st["max_ms"], "completed": st["completed"], "wall_s": wall,
+            "aggregate_cgs_per_s": st["cell_gene_steps"] / wall, "busy_cgs_per_s": st["cell_gene_steps"] / max(st["busy_s"], 1e-9)}
+
+
+def run_copy(a) -> dict:
+    """Plain device-to-device copy of known size (hipMemcpy's blit kernel), used
+    to scale the FETCH_SIZE / WRITE_SIZE hardware counters to bytes."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = a.bytes
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) == 0     # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    return {"workload": "device copy", "bytes": nbytes, "copies": 3}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["g20", "g256", "batch", "serve", "copy"], default="g20")
+    ap.add_argument("--cells", type=int, default=None, help="override cell count (testing only)")
+    ap.add_argument("--tile", type=int, default=0, help="genes per lane for the FP32 stepper (2, 4, 8, 16)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--subset", type=int, default=256, help="g256: cells replayed by the FP64 oracle")
+    ap.add_argument("--networks", type=int, default=4096, help="batch: number of networks")
+    ap.add_argument("--T", type=float, default=None, help="batch: horizon")
+    ap.add_argument("--qps", type=float, default=4.0, help="serve: arrival rate")
+    ap.add_argument("--requests", type=int, default=60, help="serve: number of requests")
+    ap.add_argument("--bytes", type=int, default=1 << 30, help="copy: bytes per copy")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(cells and networks would shard across GPUs with no exchange; see DESIGN.md).",
+              file=sys.stderr)
+        return 2
+    if a.workload == "copy":
+        print(json.dumps(run_copy(a)))
+        return 0
+
+    import scrna_amd as sc
+
+    if a.tile:
+        sc.default_context().set_tile(1, min(256, 16 * a.tile), a.tile)
+    runner = {"g20": run_g20, "g256": run_g256, "batch": run_batch, "serve": run_serve}[a.workload]
+    print(json.dumps(runner(a, sc)))
     return 0
 
 
