@@ -125,6 +125,12 @@ def run_g20(a, sc) -> dict:
     k_ms = float(np.mean(kernel_ms))
     census = valu_census(net)
     achieved = census * cgs_per_run / (k_ms * 1e-3) / 1e12
+    # HBM bytes per stepper launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
+    # workload (copy-calibrated); null when that measurement is absent or for another cell count.
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "cle_stepper_g20_traffic.json")
+    if os.path.exists(tpath) and wl["cells"] == 100_000:
+        traffic = json.load(open(tpath))["traffic_bytes"]
     out = {
         "metric": "cell*gene*steps/s (GPU CLE, FP32 state)",
         "value": value,
@@ -144,7 +150,7 @@ def run_g20(a, sc) -> dict:
                    "snapshot_every": wl["snapshot_every"], "capture": "LogNormal(0.1, 0.3)-Binomial",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "Tlane-op/s",
-                     "frac": achieved / VALU_PEAK_TOPS, "traffic": None,
+                     "frac": achieved / VALU_PEAK_TOPS, "traffic": traffic,
                      "kernel": "cle_stepper_kernel", "kernel_ms": k_ms, "census_ops_per_cgs": census,
                      "fp32_equiv": {"achieved": 4 * achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s"}},
     }
@@ -172,16 +178,16 @@ def run_g256(a, sc) -> dict:
     out = {"workload": "G=256 additive network, 25% dense A, 8 AND + 8 OR, 65536 cells, 2000 steps dt=0.005",
            "cells": cells, "steps": steps, "results": {}}
     subsets = {}
-    for prec in ("fp32", "bf16", "bf16x3"):
+    for prec in ("fp32_unfused", "fp32", "bf16", "bf16x3"):
         plan = sc.Plan([net], sc.CLE(dt), kin, cell_num=cells, T=T, readout="both", seed=777, precision=prec)
         el, kms = timed_runs(plan, a.steps, a.warmup)
         cgs = float(net.G) * cells * steps
         k_ms = float(np.mean(kms))
-        census = valu_census(net, dense=prec != "fp32")
+        census = valu_census(net, dense=prec in ("bf16", "bf16x3"))
         r = {"cgs_per_s": cgs * a.steps / el, "kernel_ms": k_ms,
              "valu_frac": census * cgs / (k_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS,
              "census_ops_per_cgs": census}
-        if prec != "fp32":
+        if prec in ("bf16", "bf16x3"):
             mfma_flops = 2.0 * net.G * cgs * (3 if prec == "bf16x3" else 1)
             r["mfma_tflops_issued"] = mfma_flops / (k_ms * 1e-3) / 1e12
         y = plan.download(0).y
@@ -206,8 +212,10 @@ def run_g256(a, sc) -> dict:
             out["results"][prec]["vs_fp64_replay"] = errs
         out["replay_cells"] = n_sub
     base = out["results"]["fp32"]["cgs_per_s"]
-    for prec in ("bf16", "bf16x3"):
+    base_ref = out["results"]["fp32_unfused"]["cgs_per_s"]
+    for prec in ("fp32", "bf16", "bf16x3"):
         out["results"][prec]["speedup_vs_fp32_fused"] = out["results"][prec]["cgs_per_s"] / base
+        out["results"][prec]["speedup_vs_fp32_reference_structured"] = out["results"][prec]["cgs_per_s"] / base_ref
     return out
 
 

@@ -51,7 +51,9 @@ typedef enum scrna_readout {
 typedef enum scrna_precision {
     SCRNA_PREC_FP32 = 0,        /* FP32 state, regulation on the VALU */
     SCRNA_PREC_BF16 = 1,        /* dense regulation on BF16 MFMA, FP32 accumulate */
-    SCRNA_PREC_BF16X3 = 2       /* dense regulation as hi*hi + hi*lo + lo*hi BF16 MFMA */
+    SCRNA_PREC_BF16X3 = 2,      /* dense regulation as hi*hi + hi*lo + lo*hi BF16 MFMA */
+    SCRNA_PREC_FP32_UNFUSED = 3 /* the reference GPU structure (ext:87-108): one kernel per
+                                   broadcast, two SGEMMs per step, state in HBM -- a baseline */
 } scrna_precision;
 
 /* GeneNetwork (network.jl:55-63) with CoopEdge / RedunEdge (network.jl:16-34).

@@ -21,7 +21,7 @@ OUT = os.path.join(HERE, "libscrna.so")
 OBJDIR = os.path.join(HERE, "_build")
 
 SOURCES = ["cle_kernels.hip", "runtime.cpp", "population.cpp", "server.cpp", "dense_kernels.hip",
-           "autotune.cpp"]
+           "autotune.cpp", "unfused.hip"]
 
 COMMON = [
     "--offload-arch=gfx950",
@@ -62,7 +62,8 @@ def build(jobs: int = 4) -> str:
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = OUT + ".tmp"
-    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-L/opt/rocm/lib", "-lrocblas",
+           "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}{r.stderr}")

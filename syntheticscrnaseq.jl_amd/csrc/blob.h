@@ -59,7 +59,8 @@ enum BlobHeader : int {
     H_OFF_BIAS,       // dense path: [G] f32 rowsum(A_neg)
     H_DENSE_LO,       // dense path: offset (bf16x8 units) of the packed W^T in the weight pool
     H_DENSE_HI,
-    H_WORDS = 48
+    H_OFF_SLOTGENE,   // [S] i32 gene of each source slot
+    H_WORDS = 64
 };
 
 enum BlobFlags : uint32_t {

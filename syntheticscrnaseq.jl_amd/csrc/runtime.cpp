@@ -165,7 +165,15 @@ int validate_inputs(const scrna_network* net, const scrna_kinetics* kin, const s
     if (run->readout < 0 || run->readout > 2) return set_error(SCRNA_ERR_INVALID, "readout must be 0, 1 or 2");
     if (run->init_mode < 0 || run->init_mode > 1) return set_error(SCRNA_ERR_INVALID, "init_mode must be 0 or 1");
     if (run->snapshot_every < 0) return set_error(SCRNA_ERR_INVALID, "snapshot_every must be >= 0");
-    if (run->precision < 0 || run->precision > 2) return set_error(SCRNA_ERR_INVALID, "unknown precision %d", run->precision);
+    if (run->precision < 0 || run->precision > 3) return set_error(SCRNA_ERR_INVALID, "unknown precision %d", run->precision);
+    if (run->precision == SCRNA_PREC_FP32_UNFUSED) {
+        bool tele = false;
+        if (net->k_on)
+            for (int g = 0; g < G; ++g) tele |= std::isfinite(net->k_on[g]) || std::isfinite(net->k_off[g]);
+        if (tele || (pop && pop->enabled) || run->injected_noise)
+            return set_error(SCRNA_ERR_UNSUPPORTED,
+                             "the reference-structured FP32 path has no telegraph, population or injected noise");
+    }
     if (run->net_id >= (1u << 28)) return set_error(SCRNA_ERR_INVALID, "net_id must be < 2^28");
     if ((int64_t)G * run->cell_num > ((int64_t)1 << 40)) return set_error(SCRNA_ERR_INVALID, "problem too large");
     if (pop && pop->enabled) {
@@ -186,10 +194,11 @@ int validate_inputs(const scrna_network* net, const scrna_kinetics* kin, const s
     return SCRNA_OK;
 }
 
-// Genes per lane: the smallest chunk that keeps NW = ceil(G/CH) <= 16 waves.
+// Default genes per lane when no autotuner rule applies: 4 (measured fastest
+// on MI355X for a G=20 telegraph network: 42.4 ms vs 49.6 ms with 2 and 46.9 ms
+// with 8 per 1e10 cell*gene*steps), larger where needed to keep NW <= 16 waves.
 static void choose_shape(int G, int* CH, int* NW) {
-    int ch = 2;
-    if (G > 32) ch = 4;
+    int ch = 4;
     if (G > 64) ch = 8;
     if (G > 128) ch = 16;
     *CH = ch;
@@ -217,7 +226,7 @@ int compile_network(const scrna_network* net, const scrna_kinetics* kin, const s
     out->CH = CH;
     out->NW = NW;
     // Dense MFMA path: the additive regulation becomes act * W^T + b.
-    const bool dense = run->precision != SCRNA_PREC_FP32;
+    const bool dense = run->precision == SCRNA_PREC_BF16 || run->precision == SCRNA_PREC_BF16X3;
     out->dense = dense;
     out->mode = run->precision;
     out->Gp = (G + 31) / 32 * 32;
@@ -406,6 +415,10 @@ int compile_network(const scrna_network* net, const scrna_kinetics* kin, const s
         b.push_back(f2w(p0));
     }
     align16();
+    b[H_OFF_SLOTGENE] = (uint32_t)b.size();
+    for (int g = 0; g < G; ++g)
+        if (slot_of[g] >= 0) b.push_back((uint32_t)g);
+    align16();
     b[H_GP] = (uint32_t)out->Gp;
     b[H_OFF_BIAS] = (uint32_t)b.size();
     for (int g = 0; g < G; ++g) b.push_back(f2w(dense ? (float)bias[g] : 0.0f));
@@ -447,6 +460,7 @@ void plan_release(scrna_plan* p) {
     if (p->ev_start) (void)hipEventDestroy(p->ev_start);
     if (p->ev_stop) (void)hipEventDestroy(p->ev_stop);
     p->ev_start = p->ev_stop = nullptr;
+    for (auto& u : p->unfused) unfused_release(u);
     if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     p->graph_exec = nullptr;
@@ -534,6 +548,25 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
     size_t w_elems = 0;
     for (int i = 0; i < n; ++i) {
         CompiledNet& cn = P->nets[i];
+        if (cn.precision == SCRNA_PREC_FP32_UNFUSED) {
+            // Reference-structured baseline: no tiles, its own per-step launch sequence.
+            UnfusedNet u;
+            u.net = i;
+            u.hill_n = (float)kins[i].n;
+            u.Kn = (float)std::pow(kins[i].K_d, kins[i].n);          // K_d^n in double, then FP32 (ext:71)
+            u.has_gates = nets[i].n_coop + nets[i].n_redun > 0;
+            const int G = cn.G;
+            u.host_A.assign(2 * (size_t)G * G, 0.0f);
+            for (int a = 0; a < G; ++a)
+                for (int b2 = 0; b2 < G; ++b2) {
+                    if (a == b2) continue;
+                    const double aeff = nets[i].A[a + (size_t)b2 * G] * nets[i].basals[a];   // network.jl:212
+                    u.host_A[a + (size_t)b2 * G] = (float)std::max(aeff, 0.0);             // A_pos
+                    u.host_A[(size_t)G * G + a + (size_t)b2 * G] = (float)std::max(-aeff, 0.0);   // A_neg
+                }
+            P->unfused.push_back(std::move(u));
+            continue;
+        }
         const bool tele = (cn.blob[H_FLAGS] & F_TELE) != 0, inj = (cn.blob[H_FLAGS] & F_INJECT) != 0;
         GKey key = cn.dense ? GKey{1, cn.mode, cn.NW, tele ? 1 : 0, inj ? 1 : 0} : GKey{0, cn.CH, cn.NW, 0, 0};
         auto& v = groups[key];
@@ -664,6 +697,12 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
     PLAN_ALLOC(d_wpool, uint16_t, w_elems);
 #undef PLAN_ALLOC
     P->n_cjobs = (int)cjobs.size();
+    for (auto& u : P->unfused) {
+        if ((rc = unfused_prepare(P.get(), u))) {
+            plan_release(P.get());
+            return rc;
+        }
+    }
     if (P->has_population && (rc = population_prepare(P.get(), state_elems, prom_elems, vol_elems))) {
         plan_release(P.get());
         return rc;
@@ -737,6 +776,8 @@ static int plan_enqueue(scrna_plan* P, hipStream_t st, bool events) {
     } else {
         int rc = launch_steps(P, st, 0, INT32_MAX);
         if (rc) return rc;
+        for (auto& u : P->unfused)
+            if ((rc = unfused_launch(P, u, st))) return rc;
     }
     if (events) HIP_TRY(hipEventRecord(P->ev_stop, st));
     HIP_TRY(launch_moments((int)P->mjobs_snap.size(), P->max_mom_G, P->d_mjobs_snap, P->d_snap, P->d_mom, st));
@@ -765,6 +806,8 @@ int plan_launch(scrna_plan* P, hipStream_t st) {
 // replay it with a single hipGraphLaunch.  No memset nodes are captured.
 int plan_make_graph(scrna_plan* P) {
     if (P->graph_exec) return SCRNA_OK;
+    if (!P->unfused.empty())
+        return set_error(SCRNA_ERR_UNSUPPORTED, "the reference-structured FP32 path is not captured into graphs");
     HIP_TRY(hipSetDevice(P->ctx->device));
     hipStream_t cs = nullptr;
     HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));

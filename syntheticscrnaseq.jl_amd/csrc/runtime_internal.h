@@ -56,6 +56,17 @@ struct CompiledNet {
     size_t off_w = 0;                // in 8-element (16-byte) units
 };
 
+// A network on the reference-structured FP32 path (unfused.hip).
+struct UnfusedNet {
+    int net = 0;
+    float hill_n = 2.0f, Kn = 1.0f;
+    bool has_gates = false;
+    std::vector<float> host_A;        // A_pos then A_neg, column-major G x G
+    float *Pn = nullptr, *den = nullptr, *act = nullptr, *rep = nullptr, *reg = nullptr, *tmp = nullptr,
+          *nz = nullptr, *Apos = nullptr, *Aneg = nullptr;
+    void* blas = nullptr;             // rocblas_handle
+};
+
 struct LaunchGroup {
     int kind = 0;                    // 0: sparse VALU stepper, 1: dense MFMA stepper
     int CH = 2, NW = 1, tile0 = 0, n_tiles = 0, nbuf = 1;
@@ -70,6 +81,7 @@ struct scrna_plan {
     scrna_ctx* ctx = nullptr;
     std::vector<scrna::CompiledNet> nets;
     std::vector<scrna::LaunchGroup> groups;
+    std::vector<scrna::UnfusedNet> unfused;
     std::vector<std::pair<void*, size_t>> allocs;
     uint64_t bytes = 0;
     double cell_gene_steps = 0.0;
@@ -122,6 +134,9 @@ void plan_release(scrna_plan* P);
 int population_launch(scrna_plan* P, hipStream_t st);
 int launch_steps(scrna_plan* P, hipStream_t st, int s_begin, int s_end);
 int plan_make_graph(scrna_plan* P);
+int unfused_prepare(scrna_plan* P, UnfusedNet& u);
+int unfused_launch(scrna_plan* P, UnfusedNet& u, hipStream_t st);
+void unfused_release(UnfusedNet& u);
 int ctx_tile_for(scrna_ctx* ctx, int G);
 int population_prepare(scrna_plan* P, size_t state_elems, size_t prom_elems, size_t vol_elems);
 int plan_alloc(scrna_plan* p, void** ptr, size_t bytes);

@@ -70,3 +70,17 @@ def test_g256_precision_moments(ctx, precision, mean_tol, var_tol):
         print(f"{precision} species {k}: max mean err {mean_err.max():.2e}, max var err {var_err.max():.2e}")
         assert mean_err.max() <= mean_tol
         assert var_err.max() <= var_tol
+
+
+def test_reference_structured_fp32_matches_oracle(ctx):
+    """The reference-structured FP32 baseline (per-step elementwise kernels and
+    two rocBLAS SGEMMs, ext:87-108) draws the same Philox stream as the fused
+    stepper, so it replays the oracle like the fused path does."""
+    rng = np.random.default_rng(31)
+    net = sc.samplers.mixed_network(12, rng, n_tf=(3, 4), targets_per_tf=(2, 5), n_and=(1, 2), n_or=(1, 2))
+    cells, dt, T, seed = 300, 0.01, 2.0, 41
+    y = sc.simulate(net, sc.CLE(dt), KIN, cell_num=cells, T=T, readout="both", seed=seed, precision="fp32_unfused")
+    ref = orc.cle(net, KIN, dt=dt, T=T, cell_num=cells, seed=seed)
+    err = relerr(y, np.concatenate([ref["m"].T, ref["p"].T], axis=1))
+    print(f"unfused max rel err {err:.3e}")
+    assert err <= 2e-4
