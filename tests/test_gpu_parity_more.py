@@ -1,0 +1,87 @@
+"""More GPU parity cases: Hill coefficients other than 2, dilution, readout
+layouts, and the statistical bands of the reference's own GPU test
+(test_gpu_parity.jl:12-30) and moment tests (test_analytical_moments.jl)."""
+import numpy as np
+import pytest
+
+import scrna_amd as sc
+from oracle import oracle as orc
+from scrna_amd import analytical as an
+
+pytestmark = pytest.mark.gpu
+
+
+def relerr(a, b):
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1.0)))
+
+
+@pytest.mark.parametrize("n,K", [(10.0, 61.47011276535837), (1.7, 8.0), (3.0, 12.0), (1.0, 5.0)])
+def test_hill_coefficients_injected(ctx, n, K):
+    """Default KineticParams use n = 10 (kinetics.jl:44); the kernel evaluates x^n as
+    exp2(n log2 x) for n != 2."""
+    rng = np.random.default_rng(int(n * 10))
+    net = sc.samplers.mixed_network(15, rng, n_tf=(3, 4), targets_per_tf=(2, 5), n_and=(1, 2), n_or=(1, 2),
+                                    basal_range=(2.0, 8.0))
+    kin = sc.KineticParams(k_t=4.0, K_d=K, n=n, mu_m=0.5, mu_p=0.2)
+    cells, dt, T = 128, 0.01, 3.0
+    steps = sc.n_steps(T, dt)
+    xi = rng.standard_normal((steps, net.G, 2, cells)).astype(np.float32)
+    y = sc.simulate(net, sc.CLE(dt), kin, cell_num=cells, T=T, readout="both", injected_noise=xi, seed=1)
+    ref = orc.cle(net, kin, dt=dt, n_steps=steps, cell_num=cells, injected=xi)
+    err = relerr(y, np.concatenate([ref["m"].T, ref["p"].T], axis=1))
+    print(f"n={n}: max rel err {err:.3e}")
+    assert err <= 2e-5
+
+
+def test_dilution_and_readouts(ctx):
+    net = sc.GeneNetwork(3, np.array([2.0, 3.0, 1.0]), np.array([[0, 2.0, 0], [0, 0, -1.0], [1.5, 0, 0]]))
+    kin = sc.KineticParams(k_t=2.0, K_d=10.0, n=2.0, mu_m=0.4, mu_p=0.1, dilution=0.05)
+    cells, dt, T = 200, 0.02, 2.0
+    steps = sc.n_steps(T, dt)
+    xi = np.random.default_rng(2).standard_normal((steps, 3, 2, cells)).astype(np.float32)
+    ref = orc.cle(net, kin, dt=dt, n_steps=steps, cell_num=cells, injected=xi)
+    for ro, want in (("mrna", ref["m"].T), ("protein", ref["p"].T),
+                     ("both", np.concatenate([ref["m"].T, ref["p"].T], axis=1))):
+        y = sc.simulate(net, sc.CLE(dt), kin, cell_num=cells, T=T, readout=ro, injected_noise=xi)
+        assert y.shape == want.shape
+        assert relerr(y, want) <= 1e-5
+
+
+def test_reference_gpu_vs_cpu_band(ctx):
+    """test_gpu_parity.jl:12-30: GPU CLE vs CPU CLE, G=1, beta=5, k_t=2, K_d=50, n=2,
+    5000 cells, T=200, dt=0.1: mean within 5%, variance within 15% (independent streams)."""
+    net = sc.GeneNetwork(1, np.array([5.0]), np.zeros((1, 1)))
+    kin = sc.KineticParams(k_t=2.0, K_d=50.0, n=2.0, mu_m=0.1, mu_p=0.2)
+    y_gpu = sc.simulate(net, sc.CLE(0.1), kin, cell_num=5000, T=200.0, readout="protein", seed=42)
+    y_cpu = orc.cle(net, kin, dt=0.1, T=200.0, cell_num=5000, seed=4242, threads=8)["p"].T
+    assert abs(y_cpu.mean() - y_gpu.mean()) / y_cpu.mean() < 0.05
+    assert abs(y_cpu.var(ddof=1) - y_gpu.var(ddof=1)) / y_cpu.var(ddof=1) < 0.15
+
+
+def test_two_stage_moments_on_gpu(ctx):
+    """test_analytical_moments.jl:110-121 bands, plus the exact Euler moments at the same dt."""
+    beta, mu_m, mu_p, k_t = 2.0, 0.1, 0.2, 1.0
+    net = sc.GeneNetwork(1, np.array([beta]), np.zeros((1, 1)))
+    kin = sc.KineticParams(k_t=k_t, K_d=50.0, n=2.0, mu_m=mu_m, mu_p=mu_p)
+    y = sc.simulate(net, sc.CLE(0.1), kin, cell_num=20000, T=500.0, readout="both", seed=11)
+    m, p = y[:, 0], y[:, 1]
+    ex = an.two_stage_moments(beta, k_t, mu_m, mu_p)
+    assert abs(m.mean() / ex["m_mean"] - 1) < 0.05
+    assert abs(m.var(ddof=1) / ex["m_var"] - 1) < 0.20
+    assert abs(p.mean() / ex["p_mean"] - 1) < 0.05
+    assert abs(p.var(ddof=1) / p.mean() / ex["p_fano"] - 1) < 0.20
+
+
+def test_telegraph_moments_on_gpu(ctx):
+    """Unregulated telegraph gene: mean against the discrete-time chain and the
+    Fano factor against analytical.jl:119 within the dt bias and sampling error."""
+    # fast switching and a high mean keep the clamp at zero out of play
+    k_on, k_off, beta, mu_m, dt = 5.0, 5.0, 200.0, 1.0, 0.005
+    net = sc.GeneNetwork(1, np.array([beta]), np.zeros((1, 1)), k_on=[k_on], k_off=[k_off])
+    kin = sc.KineticParams(k_t=1.0, K_d=10.0, n=2.0, mu_m=mu_m, mu_p=1.0)
+    y = sc.simulate(net, sc.CLE(dt), kin, cell_num=100000, T=20.0, readout="mrna", seed=5)[:, 0]
+    want_mean = an.telegraph_euler_mean(beta, k_on, k_off, mu_m, dt)
+    assert abs(y.mean() / want_mean - 1) < 0.02
+    fano = y.var() / y.mean()
+    want_fano = an.telegraph_fano(k_on / mu_m, k_off / mu_m, beta / mu_m)
+    assert abs(fano / want_fano - 1) < 0.08
