@@ -32,27 +32,38 @@ for _p in (os.path.join(ROOT, "syntheticscrnaseq.jl_amd"), ROOT):
 
 import numpy as np  # noqa: E402
 
-# MI355X: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz = 39.3e12 VALU lane-ops/s
-# (the published 157.3 TFLOP/s FP32 peak is this x 2 (FMA) x 2 (packed)).
-VALU_PEAK_TOPS = 39.3216
+# MI355X published peaks: FP32 vector 157.3 TFLOP/s (256 CUs x 4 SIMD x 32 FP32
+# lanes per clock x 2 (FMA) x 2.4 GHz), dense BF16 MFMA 2.5 PFLOP/s.
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
+
+# gfx950 VALU issue cost of each instruction class relative to one FP32 FMA,
+# measured by syntheticscrnaseq.jl_amd/probe_valu_rates (8 independent chains
+# per lane, 8 waves per SIMD):
+COST_FP32 = 1.0       # v_add/mul/fma/max_f32
+COST_INT = 1.66       # 32-bit integer and logic (xor, add, shifts, compares)
+COST_MUL32 = 1.54     # one 32x32-bit product (v_mad_u64_u32, v_mul_hi_u32)
+COST_TRANS = 3.63     # v_log/exp/sqrt/rcp/sin/cos_f32
 
 
 def log(msg: str) -> None:
     print(f"# {msg}", file=sys.stderr, flush=True)
 
 
-def valu_census(net, dense: bool = False) -> float:
-    """Algorithmic VALU lane-ops per cell*gene*step for a network (fixed rule).
+def valu_census(net, dense: bool = False) -> dict:
+    """Algorithmic VALU work per cell*gene*step of a network (fixed operation
+    counts), priced in FP32-FMA issue units with the measured gfx950 costs.
 
-    E = 36 FP32 ops for the update (cle.jl:79-120, every +,-,*,/,max,sqrt,pow = 1);
-    R = 2 * additive in-edges per gene (sparse path; 0 on the MFMA path) +
-        3 * gate source incidences per gene;
-    telegraph = 3 ops; quarter-rate surcharge +3 for each of 2 sqrt and 1 rcp;
-    Box-Muller for one normal pair (log, sqrt, sin, cos at 4 each + 5) = 21;
-    Philox4x32-10 = 138 ops per call (20 32x32 products at 4 + 58 xor/add),
-    0.5 call per gene for the two normals, +0.25 with telegraph switching.
+    Operation counts:
+      update E = 36 ops (cle.jl:79-120, every +,-,*,/,max,sqrt,pow = 1), of
+        which 2 sqrt and 1 reciprocal are transcendental and 33 FP32;
+      regulation R = 2 * additive in-edges per gene (0 on the MFMA path, whose
+        contraction is MFMA work) + 3 * gate source incidences per gene, FP32;
+      Box-Muller for the pair of normals: log, sqrt, sin, cos + 5 FP32;
+      Philox4x32-10: 20 32x32-bit products + 58 xor/add per call, 0.5 call per
+        gene (+0.25 with telegraph switching, which adds 3 integer ops).
+    Returns the class counts, the FMA-equivalent total and the FP32-equivalent
+    flops (2 per FMA-equivalent op).
     """
     G = net.G
     A = np.array(net.interactions)
@@ -60,10 +71,13 @@ def valu_census(net, dense: bool = False) -> float:
     d_add = 0.0 if dense else np.count_nonzero(A) / G
     inc = sum(len(e.sources) for e in list(net.cooperative) + list(net.redundant)) / G
     tele = bool(np.any(np.isfinite(net.k_on)))
-    ops = 36 + 2 * d_add + 3 * inc + 9 + 21 + 138 * 0.5
-    if tele:
-        ops += 3 + 138 * 0.25
-    return float(ops)
+    calls = 0.5 + (0.25 if tele else 0.0)
+    counts = {"fp32": 33 + 2 * d_add + 3 * inc + 5, "trans": 3 + 4,
+              "int": 58 * calls + (3 if tele else 0), "mul32": 20 * calls}
+    fma_eq = (counts["fp32"] * COST_FP32 + counts["trans"] * COST_TRANS + counts["int"] * COST_INT
+              + counts["mul32"] * COST_MUL32)
+    return {"counts": {k: round(v, 3) for k, v in counts.items()}, "fma_equiv_ops": fma_eq,
+            "fp32_equiv_flops": 2.0 * fma_eq}
 
 
 def cpu_baseline(wl, target_s: float = 12.0) -> dict:
@@ -124,7 +138,7 @@ def run_g20(a, sc) -> dict:
     value = cgs_per_run * a.steps / el
     k_ms = float(np.mean(kernel_ms))
     census = valu_census(net)
-    achieved = census * cgs_per_run / (k_ms * 1e-3) / 1e12
+    achieved = census["fp32_equiv_flops"] * cgs_per_run / (k_ms * 1e-3) / 1e12
     # HBM bytes per stepper launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
     # workload (copy-calibrated); null when that measurement is absent or for another cell count.
     traffic = None
@@ -149,10 +163,11 @@ def run_g20(a, sc) -> dict:
                    "G": net.G, "cells": wl["cells"], "sim_steps": steps_per_run,
                    "snapshot_every": wl["snapshot_every"], "capture": "LogNormal(0.1, 0.3)-Binomial",
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "Tlane-op/s",
-                     "frac": achieved / VALU_PEAK_TOPS, "traffic": traffic,
-                     "kernel": "cle_stepper_kernel", "kernel_ms": k_ms, "census_ops_per_cgs": census,
-                     "fp32_equiv": {"achieved": 4 * achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s"}},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s (FP32-FMA-equivalent VALU issue)",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
+                     "kernel": "cle_stepper_kernel", "kernel_ms": k_ms,
+                     "census_per_cgs": census},
     }
     if a.tile:
         out["config"]["genes_per_lane"] = a.tile
@@ -185,8 +200,8 @@ def run_g256(a, sc) -> dict:
         k_ms = float(np.mean(kms))
         census = valu_census(net, dense=prec in ("bf16", "bf16x3"))
         r = {"cgs_per_s": cgs * a.steps / el, "kernel_ms": k_ms,
-             "valu_frac": census * cgs / (k_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS,
-             "census_ops_per_cgs": census}
+             "valu_frac": census["fp32_equiv_flops"] * cgs / (k_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+             "census_per_cgs": census}
         if prec in ("bf16", "bf16x3"):
             mfma_flops = 2.0 * net.G * cgs * (3 if prec == "bf16x3" else 1)
             r["mfma_tflops_issued"] = mfma_flops / (k_ms * 1e-3) / 1e12

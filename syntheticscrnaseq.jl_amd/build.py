@@ -22,6 +22,8 @@ OBJDIR = os.path.join(HERE, "_build")
 
 SOURCES = ["cle_kernels.hip", "runtime.cpp", "population.cpp", "server.cpp", "dense_kernels.hip",
            "autotune.cpp", "unfused.hip"]
+# Stand-alone measurement programs (not linked into the library).
+PROBES = ["probe_valu_rates.hip"]
 
 COMMON = [
     "--offload-arch=gfx950",
@@ -68,6 +70,12 @@ def build(jobs: int = 4) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}{r.stderr}")
     os.replace(tmp, OUT)
+    for src in PROBES:
+        exe = os.path.join(HERE, os.path.splitext(src)[0])
+        cmd = [hipcc(), *COMMON, os.path.join(CSRC, src), "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc failed on {src}:\n{r.stdout}{r.stderr}")
     return OUT
 
 

@@ -25,12 +25,27 @@ struct MomentJob {
     int32_t cells, G;
 };
 
+// Segmented schedule of the FP32 stepper: a launch over n_tiles tiles with
+// n_seg > 1 runs n_tiles * n_seg workgroups; one int block per launch group
+// holds the ticket counter, an error word and one done-flag per tile.
+enum SchedWord : int { SCHED_TICKET = 0, SCHED_ERROR = 1, SCHED_DONE = 2 };
+inline size_t sched_words(int n_tiles) { return (size_t)SCHED_DONE + (size_t)n_tiles; }
+
+// Static LDS of the stepper kernel, beside its dynamic Hill buffers.
+constexpr size_t kStepperStaticLds = 64;
 size_t stepper_lds_bytes(int S, int nbuf);
 hipError_t set_stepper_lds_limit(size_t bytes);
+// Resident workgroups per CU for the stepper at this shape (0 on error).
+int stepper_blocks_per_cu(int CH, int NW, size_t lds_bytes);
+// sched == nullptr: one workgroup per tile over the whole step range.
+// sched != nullptr: n_seg segments per tile, ordered by tickets; sched must be
+// reset (launch_sched_reset) before every such launch.  wait_ticks bounds a
+// segment's wait for its predecessor, in wall-clock ticks.
 hipError_t launch_stepper(int CH, int NW, int n_tiles, const uint32_t* blobs, const Tile* tiles,
                           float* state_pool, float* snap_pool, uint8_t* prom_pool, double* vol_pool,
                           const float* inj_pool, int s_begin, int s_end, int nbuf,
-                          size_t lds_bytes, hipStream_t stream);
+                          size_t lds_bytes, int* sched, int n_seg, long long wait_ticks, hipStream_t stream);
+hipError_t launch_sched_reset(int* sched, int n_words, hipStream_t stream);
 hipError_t launch_capture_f32(int n_jobs, int64_t max_elems, const CaptureJob* jobs, const float* src,
                               int32_t* dst, hipStream_t stream);
 hipError_t launch_capture_f64(int n_jobs, int64_t max_elems, const CaptureJob* jobs, const double* src,

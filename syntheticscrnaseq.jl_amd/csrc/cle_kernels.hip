@@ -34,11 +34,13 @@ __device__ __forceinline__ uint64_t u64_at(const uint32_t* B, int i) {
 // Hill pair for one protein value: act = x^n/(1+x^n), rep = 1/(1+x^n) with
 // x = p/K (network.jl:112-125; same quantities as cle.jl:79-82 divided
 // through by K^n, which keeps x^n finite for large p).
-__device__ __forceinline__ void hill_pair(float p, float invK, float hill_n, bool hill2,
-                                          float& act, float& rep) {
+// HILL2 selects n = 2 at compile time; callers branch on the (uniform) flag
+// outside their gene loops so that the n = 2 path never evaluates log/exp.
+template <bool HILL2>
+__device__ __forceinline__ void hill_pair(float p, float invK, float hill_n, float& act, float& rep) {
     const float x = p * invK;
     float xn;
-    if (hill2) {
+    if (HILL2) {
         xn = x * x;
     } else {
         xn = __builtin_amdgcn_exp2f(hill_n * __builtin_amdgcn_logf(x));
@@ -48,22 +50,84 @@ __device__ __forceinline__ void hill_pair(float p, float invK, float hill_n, boo
     act = xn * rep;
 }
 
+// Hill values of a wave's CH genes that are regulation sources -> LDS rows
+// [2*slot] (act) and [2*slot+1] (rep), one column per lane.
+template <int CH, bool HILL2>
+__device__ __forceinline__ void hill_to_lds(const float (&p)[CH], int g0, int G, const int32_t* __restrict__ slot_of,
+                                            float invK, float hill_n, float* __restrict__ H, int lane) {
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+        const int g = g0 + k;
+        if (g < G) {
+            const int slot = slot_of[g];
+            if (slot >= 0) {
+                float act, rep;
+                hill_pair<HILL2>(p[k], invK, hill_n, act, rep);
+                H[(2 * slot) * 64 + lane] = act;
+                H[(2 * slot + 1) * 64 + lane] = rep;
+            }
+        }
+    }
+}
+
 template <int CH>
 __global__ void __launch_bounds__(1024)
 cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ tiles,
                    float* __restrict__ state_pool, float* __restrict__ snap_pool,
                    uint8_t* __restrict__ prom_pool, double* __restrict__ vol_pool,
                    const float* __restrict__ inj_pool, int s_begin, int s_end,
-                   int nbuf) {
+                   int nbuf, int* __restrict__ sched, int n_tiles, int n_seg, long long wait_ticks) {
     extern __shared__ float lds[];
-    const Tile tile = tiles[blockIdx.x];
+    __shared__ int sh_ticket, sh_fail;
+    int ti = blockIdx.x, seg = 0;
+    if (sched != nullptr) {
+        // Segmented schedule: ticket t (taken in start order) is segment t / n_tiles
+        // of tile t % n_tiles, so the segment this one continues from took an
+        // earlier ticket and is already resident or finished.
+        if (threadIdx.x == 0) sh_ticket = atomicAdd(&sched[SCHED_TICKET], 1);
+        __syncthreads();
+        seg = sh_ticket / n_tiles;
+        ti = sh_ticket - seg * n_tiles;
+    }
+    const Tile tile = tiles[ti];
     const uint32_t* __restrict__ B = blobs + tile.blob;
     const int G = (int)B[H_G];
-    const int s_start = max(s_begin, tile.s0);
-    const bool resume = s_start > 0;       // later chunks continue from the state pool
+    const int n_steps = (int)B[H_NSTEPS];
+    // Step range of this launch, clamped to the tile and to the network's
+    // horizon (networks of a batch may differ in horizon), then this segment's share.
+    const int r_lo = max(s_begin, tile.s0);
+    const int r_hi = max(r_lo, min(min(s_end, tile.s1), n_steps));
+    const int s_start = r_lo + (int)(((long long)(r_hi - r_lo) * seg) / n_seg);
+    const int s_stop = r_lo + (int)(((long long)(r_hi - r_lo) * (seg + 1)) / n_seg);
+    if (sched != nullptr && seg > 0) {
+        // Wait for the previous segment of this tile (release/acquire at agent
+        // scope: the tiles' states cross XCD L2s).  The wait always ends because
+        // that segment holds an earlier ticket; the clock bound only turns a
+        // scheduling bug into a reported error instead of a hang.
+        if (threadIdx.x == 0) {
+            int fail = 0;
+            const long long t0 = wall_clock64();
+            while (__hip_atomic_load(&sched[SCHED_DONE + ti], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < seg) {
+                __builtin_amdgcn_s_sleep(16);
+                if (wall_clock64() - t0 > wait_ticks) {
+                    fail = 1;
+                    atomicOr(&sched[SCHED_ERROR], 1);
+                    break;
+                }
+            }
+            sh_fail = fail;
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (sh_fail) {
+            if (threadIdx.x == 0) __hip_atomic_store(&sched[SCHED_DONE + ti], seg + 1, __ATOMIC_RELEASE,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+    }
+    const bool resume = s_start > 0;       // later chunks and segments continue from the state pool
     const int S = (int)B[H_S];
     const uint32_t flags = B[H_FLAGS];
-    const int n_steps = (int)B[H_NSTEPS];
     const int snap_every = (int)B[H_SNAP_EVERY];
     const int C = (int)B[H_CELLS];
     const uint32_t k0 = B[H_SEED_LO], k1 = B[H_SEED_HI];
@@ -104,8 +168,8 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
 
     float m[CH], p[CH];
     uint32_t on = 0;                                       // promoter bit k = gene g0+k
+    uint32_t sw_mask = 0;                                  // bit k: gene g0+k switches
     double V = 1.0;                                       // cell volume (population runs)
-    bool any_switch = false;
 #pragma unroll
     for (int k = 0; k < CH; ++k) {
         const int g = g0 + k;
@@ -113,7 +177,7 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
         p[k] = 0.0f;
         if (g < G) {
             const uint32_t tf = telew[g * 8 + T_FLAGS];
-            any_switch |= (tf & 1u) != 0;
+            sw_mask |= (tf & 1u) << k;
             if (resume) {
                 m[k] = state_pool[off_state + (size_t)g * C + cell];
                 p[k] = state_pool[off_state + (size_t)(G + g) * C + cell];
@@ -133,10 +197,10 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
         }
     }
     if (pop) V = vol_pool[off_vol + cell];   // written by the population init kernel or a previous phase
-    any_switch = tele && any_switch;
+    if (!tele) sw_mask = 0;
+    const bool any_switch = sw_mask != 0;
 
     const int rows = 2 * S;
-    const int s_stop = min(min(s_end, tile.s1), n_steps);               // networks of a batch may differ in horizon
     for (int s = s_start; s < s_stop; ++s) {
         float* __restrict__ H = lds + (nbuf == 2 ? (s & 1) * rows * 64 : 0);
 
@@ -146,17 +210,25 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
             for (int k = 0; k < CH; k += 4) {
                 const int g = g0 + k;
                 if (g < G) {
-                    const U4 u = philox4x32_10((uint32_t)cell, (uint32_t)s, (uint32_t)(g >> 2), c3s, k0, k1);
+                    const U4 u = philox4x32_10_uk((uint32_t)cell, (uint32_t)s, (uint32_t)(g >> 2), c3s, k0, k1);
                     const uint32_t uu[4] = {u.x, u.y, u.z, u.w};
 #pragma unroll
                     for (int j = 0; j < 4 && k + j < CH; ++j) {
-                        const int gg = g + j;
-                        if (gg < G && (telew[gg * 8 + T_FLAGS] & 1u)) {
-                            const uint64_t x = (uint64_t)uu[(g & 3) + j];
+                        if ((sw_mask >> (k + j)) & 1u) {
+                            // Both thresholds are uniform scalar loads; the lane's promoter
+                            // state picks one (readfirstlane keeps the selection on values,
+                            // not on a per-lane address).  thr <= 2^32, so x < thr is
+                            // hi != 0 or x < lo.
+                            const uint32_t* t = telew + (g + j) * 8;
+                            const uint32_t on_lo = __builtin_amdgcn_readfirstlane(t[T_ON_LO]);
+                            const uint32_t on_hi = __builtin_amdgcn_readfirstlane(t[T_ON_HI]);
+                            const uint32_t off_lo = __builtin_amdgcn_readfirstlane(t[T_OFF_LO]);
+                            const uint32_t off_hi = __builtin_amdgcn_readfirstlane(t[T_OFF_HI]);
+                            const uint32_t x = uu[(g & 3) + j];
                             const bool cur = (on >> (k + j)) & 1u;
-                            const uint64_t thr = cur ? u64_at(telew, gg * 8 + T_OFF_LO)
-                                                     : u64_at(telew, gg * 8 + T_ON_LO);
-                            const bool flip = x < thr;
+                            const uint32_t lo = cur ? off_lo : on_lo;
+                            const uint32_t hi = cur ? off_hi : on_hi;
+                            const bool flip = (hi != 0u) | (x < lo);
                             on ^= (uint32_t)flip << (k + j);
                         }
                     }
@@ -165,18 +237,10 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
         }
 
         // Hill values of this wave's source genes -> LDS (cle.jl:79-82).
-#pragma unroll
-        for (int k = 0; k < CH; ++k) {
-            const int g = g0 + k;
-            if (g < G) {
-                const int slot = slot_of[g];
-                if (slot >= 0) {
-                    float act, rep;
-                    hill_pair(p[k], invK, hill_n, hill2, act, rep);
-                    H[(2 * slot) * 64 + lane] = act;
-                    H[(2 * slot + 1) * 64 + lane] = rep;
-                }
-            }
+        if (hill2) {
+            hill_to_lds<CH, true>(p, g0, G, slot_of, invK, hill_n, H, lane);
+        } else {
+            hill_to_lds<CH, false>(p, g0, G, slot_of, invK, hill_n, H, lane);
         }
         __syncthreads();
 
@@ -195,7 +259,7 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
                         xp[j] = inj_pool[off_inj + (base + 1) * C + cell] * sqrt_dt;
                     }
                 } else {
-                    const U4 u = philox4x32_10((uint32_t)cell, (uint32_t)s, (uint32_t)(g >> 1), c3n, k0, k1);
+                    const U4 u = philox4x32_10_uk((uint32_t)cell, (uint32_t)s, (uint32_t)(g >> 1), c3n, k0, k1);
                     normal_pair_scaled(u.x, u.y, neg2ln2dt, xm[0], xp[0]);
                     normal_pair_scaled(u.z, u.w, neg2ln2dt, xm[1], xp[1]);
                 }
@@ -270,6 +334,22 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
         }
         if (pop && wave == 0) vol_pool[off_vol + cell] = V;
     }
+    if (sched != nullptr) {
+        // Publish this segment's state: every wave's stores complete at agent
+        // scope before the flag that releases the next segment.
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (threadIdx.x == 0)
+            __hip_atomic_store(&sched[SCHED_DONE + ti], seg + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Resets one segmented-schedule block (ticket, error word, per-tile done
+// flags) before a launch; a kernel rather than a memset so that the launch
+// sequence can be captured in a graph.
+__global__ void __launch_bounds__(256) sched_reset_kernel(int* __restrict__ sched, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) sched[i] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -483,33 +563,53 @@ size_t stepper_lds_bytes(int S, int nbuf) { return (size_t)nbuf * 2 * (size_t)S 
 hipError_t launch_stepper(int CH, int NW, int n_tiles, const uint32_t* blobs, const Tile* tiles,
                           float* state_pool, float* snap_pool, uint8_t* prom_pool, double* vol_pool,
                           const float* inj_pool, int s_begin, int s_end, int nbuf,
-                          size_t lds_bytes, hipStream_t stream) {
+                          size_t lds_bytes, int* sched, int n_seg, long long wait_ticks, hipStream_t stream) {
     if (n_tiles <= 0) return hipSuccess;
-    const dim3 grid(n_tiles), block(64 * NW);
+    if (sched == nullptr) n_seg = 1;
+    if (n_seg < 1 || (long long)n_tiles * n_seg > INT32_MAX) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)(n_tiles * n_seg)), block(64 * NW);
+#define SCRNA_STEPPER_CASE(N)                                                                                  \
+    case N:                                                                                                    \
+        hipLaunchKernelGGL(cle_stepper_kernel<N>, grid, block, lds_bytes, stream, blobs, tiles, state_pool,   \
+                           snap_pool, prom_pool, vol_pool, inj_pool, s_begin, s_end, nbuf, sched, n_tiles, n_seg, \
+                           wait_ticks);                                                                        \
+        break;
     switch (CH) {
-        case 2:
-            hipLaunchKernelGGL(cle_stepper_kernel<2>, grid, block, lds_bytes, stream, blobs, tiles, state_pool,
-                               snap_pool, prom_pool, vol_pool, inj_pool, s_begin, s_end, nbuf);
-            break;
-        case 4:
-            hipLaunchKernelGGL(cle_stepper_kernel<4>, grid, block, lds_bytes, stream, blobs, tiles, state_pool,
-                               snap_pool, prom_pool, vol_pool, inj_pool, s_begin, s_end, nbuf);
-            break;
-        case 8:
-            hipLaunchKernelGGL(cle_stepper_kernel<8>, grid, block, lds_bytes, stream, blobs, tiles, state_pool,
-                               snap_pool, prom_pool, vol_pool, inj_pool, s_begin, s_end, nbuf);
-            break;
-        case 16:
-            hipLaunchKernelGGL(cle_stepper_kernel<16>, grid, block, lds_bytes, stream, blobs, tiles, state_pool,
-                               snap_pool, prom_pool, vol_pool, inj_pool, s_begin, s_end, nbuf);
-            break;
+        SCRNA_STEPPER_CASE(2)
+        SCRNA_STEPPER_CASE(4)
+        SCRNA_STEPPER_CASE(8)
+        SCRNA_STEPPER_CASE(16)
         default:
             return hipErrorInvalidValue;
     }
+#undef SCRNA_STEPPER_CASE
     return hipGetLastError();
 }
 
+hipError_t launch_sched_reset(int* sched, int n_words, hipStream_t stream) {
+    if (n_words <= 0) return hipSuccess;
+    hipLaunchKernelGGL(sched_reset_kernel, dim3((unsigned)((n_words + 255) / 256)), dim3(256), 0, stream, sched,
+                       n_words);
+    return hipGetLastError();
+}
+
+int stepper_blocks_per_cu(int CH, int NW, size_t lds_bytes) {
+    const void* f = nullptr;
+    switch (CH) {
+        case 2: f = (const void*)cle_stepper_kernel<2>; break;
+        case 4: f = (const void*)cle_stepper_kernel<4>; break;
+        case 8: f = (const void*)cle_stepper_kernel<8>; break;
+        case 16: f = (const void*)cle_stepper_kernel<16>; break;
+        default: return 0;
+    }
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 64 * NW, lds_bytes) != hipSuccess) return 0;
+    return nb;
+}
+
 hipError_t set_stepper_lds_limit(size_t bytes) {
+    // The kernel's own static LDS (schedule ticket and flag) shares the budget.
+    bytes = bytes > kStepperStaticLds ? bytes - kStepperStaticLds : bytes;
     hipError_t e = hipSuccess;
     const void* fns[] = {(const void*)cle_stepper_kernel<2>, (const void*)cle_stepper_kernel<4>,
                          (const void*)cle_stepper_kernel<8>, (const void*)cle_stepper_kernel<16>};

@@ -46,6 +46,17 @@ __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c
     return U4{c0, c1, c2, c3};
 }
 
+// The same generator for a key that is wave-uniform (the run seed).  The empty
+// asm makes the key opaque at each call, so the ten round keys are recomputed
+// on the scalar unit inside the call instead of being hoisted out of a long
+// loop, where 20 live SGPRs push other uniforms into VGPR lanes and every
+// reload costs a VALU v_readlane.
+__device__ __forceinline__ U4 philox4x32_10_uk(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                                uint32_t k0, uint32_t k1) {
+    asm volatile("" : "+s"(k0), "+s"(k1));
+    return philox4x32_10(c0, c1, c2, c3, k0, k1);
+}
+
 // Box-Muller on two u32.  The radius uniform ((a>>9)+1/2)*2^-23 and the
 // angle (b>>8)*2^-24 (revolutions) are exact in float, so the CPU oracle's
 // double transform sees the same uniforms.  neg2dt = -2*ln(2)*scale^2

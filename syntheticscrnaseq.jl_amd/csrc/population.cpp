@@ -57,7 +57,7 @@ int population_launch(scrna_plan* P, hipStream_t st) {
     HIP_TRY_P(launch_pop_init(n_tiles, P->d_blobs, P->d_tiles, P->d_vol, P->d_vinit, st));
     for (int64_t s0 = 0; s0 < n_max || s0 == 0; s0 += I) {
         const int64_t s1 = std::min<int64_t>(s0 + I, n_max);
-        int rc = launch_steps(P, st, (int)s0, (int)s1);
+        int rc = launch_steps(P, st, (int)s0, (int)s1, false);
         if (rc) return rc;
         if (s1 > s0 && s1 % I == 0) {   // cle.jl:240 (step % interval == 0, 1-based step = s1)
             HIP_TRY_P(launch_pop_divide(n_tiles, P->d_blobs, P->d_tiles, P->d_state, P->d_prom, P->d_vol,

@@ -545,6 +545,7 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
     };
     std::map<GKey, std::vector<Tile>> groups;
     std::map<GKey, std::pair<int, int>> group_SG;   // max S, max Gp
+    std::map<GKey, int64_t> group_steps;            // longest horizon
     size_t w_elems = 0;
     for (int i = 0; i < n; ++i) {
         CompiledNet& cn = P->nets[i];
@@ -573,6 +574,7 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
         auto& sg = group_SG[key];
         sg.first = std::max(sg.first, cn.S);
         sg.second = std::max(sg.second, cn.Gp);
+        group_steps[key] = std::max(group_steps[key], cn.n_steps);
         const int per = cn.dense ? 16 : 64;
         for (int64_t c0 = 0; c0 < cn.cells; c0 += per) {
             Tile t;
@@ -605,6 +607,12 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
             lg.CH = kv.first.a;
             lg.nbuf = stepper_lds_bytes(S, 2) <= ctx->lds_limit_soft ? 2 : 1;
             lg.lds = stepper_lds_bytes(S, lg.nbuf);
+            const long long resident = (long long)stepper_blocks_per_cu(lg.CH, lg.NW, lg.lds) * ctx->num_cu;
+            lg.nseg = choose_segments(lg.n_tiles, resident, group_steps[kv.first]);
+            if (lg.nseg > 1) {
+                lg.sched_off = P->sched_words;
+                P->sched_words += sched_words(lg.n_tiles);
+            }
         } else {
             lg.mode = kv.first.a;
             lg.tele = kv.first.tele != 0;
@@ -612,7 +620,7 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
             lg.nbuf = 2;
             lg.lds = dense_lds_bytes(group_SG[kv.first].second, S, lg.mode);
         }
-        if (lg.lds > ctx->lds_max)
+        if (lg.lds + (lg.kind == 0 ? kStepperStaticLds : 0) > ctx->lds_max)
             return set_error(SCRNA_ERR_UNSUPPORTED, "%d source genes need %zu B of LDS per workgroup (max %zu)", S,
                              lg.lds, ctx->lds_max);
         all_tiles.insert(all_tiles.end(), kv.second.begin(), kv.second.end());
@@ -695,7 +703,9 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
     PLAN_ALLOC(d_mjobs_snap, MomentJob, P->mjobs_snap.size());
     PLAN_ALLOC(d_mjobs_state, MomentJob, P->mjobs_state.size());
     PLAN_ALLOC(d_wpool, uint16_t, w_elems);
+    PLAN_ALLOC(d_sched, int, P->sched_words);
 #undef PLAN_ALLOC
+    P->wait_ticks = 300000LL * ctx->wall_khz;   // 300 s
     P->n_cjobs = (int)cjobs.size();
     for (auto& u : P->unfused) {
         if ((rc = unfused_prepare(P.get(), u))) {
@@ -752,14 +762,43 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
     return SCRNA_OK;
 }
 
+// Segments per tile for a whole-horizon FP32 stepper launch.  One workgroup
+// per tile runs in ceil(n_tiles / resident) rounds of a full horizon, and the
+// last round is often nearly empty; k segments per tile cut the quantum to
+// 1/k of the horizon.  Pick k in 1..8 minimising ceil(n_tiles*k/resident)/k
+// (smallest k on ties), with at least 64 steps per segment.
+int choose_segments(int n_tiles, long long resident, int64_t steps) {
+    if (resident <= 0 || n_tiles <= resident) return 1;
+    int best = 1;
+    double best_cost = (double)((n_tiles + resident - 1) / resident);
+    for (int k = 2; k <= 8 && steps / k >= 64; ++k) {
+        const long long work = (long long)n_tiles * k;
+        const double cost = (double)((work + resident - 1) / resident) / k;
+        if (cost < best_cost - 1e-9) {
+            best = k;
+            best_cost = cost;
+        }
+    }
+    return best;
+}
+
 // Run steps [s_begin, s_end) of every network (clamped to each horizon).
 // A tile resumes from the state pool exactly when its first step is > 0.
-int launch_steps(scrna_plan* P, hipStream_t st, int s_begin, int s_end) {
+// `segmented` enables the segmented schedule of the FP32 stepper groups.
+int launch_steps(scrna_plan* P, hipStream_t st, int s_begin, int s_end, bool segmented) {
     for (const LaunchGroup& lg : P->groups) {
-        if (lg.kind == 0)
+        if (lg.kind == 0) {
+            int* sched = nullptr;
+            int nseg = 1;
+            if (segmented && lg.nseg > 1) {
+                sched = P->d_sched + lg.sched_off;
+                nseg = lg.nseg;
+                HIP_TRY(launch_sched_reset(sched, (int)sched_words(lg.n_tiles), st));
+            }
             HIP_TRY(launch_stepper(lg.CH, lg.NW, lg.n_tiles, P->d_blobs, P->d_tiles + lg.tile0, P->d_state,
-                                   P->d_snap, P->d_prom, P->d_vol, P->d_inj, s_begin, s_end, lg.nbuf, lg.lds, st));
-        else
+                                   P->d_snap, P->d_prom, P->d_vol, P->d_inj, s_begin, s_end, lg.nbuf, lg.lds, sched,
+                                   nseg, P->wait_ticks, st));
+        } else
             HIP_TRY(launch_dense(lg.mode, lg.tele, lg.inject, lg.NW, lg.n_tiles, P->d_blobs, P->d_tiles + lg.tile0,
                                  P->d_wpool, P->d_state, P->d_snap, P->d_prom, P->d_inj, s_begin, s_end, lg.lds,
                                  st));
@@ -774,7 +813,7 @@ static int plan_enqueue(scrna_plan* P, hipStream_t st, bool events) {
         int rc = population_launch(P, st);
         if (rc) return rc;
     } else {
-        int rc = launch_steps(P, st, 0, INT32_MAX);
+        int rc = launch_steps(P, st, 0, INT32_MAX, true);
         if (rc) return rc;
         for (auto& u : P->unfused)
             if ((rc = unfused_launch(P, u, st))) return rc;
@@ -917,6 +956,9 @@ int scrna_ctx_create(int32_t device, uint64_t hbm_budget_bytes, scrna_ctx** out)
     C->lds_max = (size_t)lds;
     C->lds_limit_soft = std::min<size_t>(C->lds_max, 64 * 1024);
     C->num_cu = prop.multiProcessorCount;
+    int wall_khz = 0;
+    if (hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && wall_khz > 0)
+        C->wall_khz = wall_khz;
     if (C->lds_max > 65536 && (set_stepper_lds_limit(C->lds_max) != hipSuccess ||
                                set_dense_lds_limit(C->lds_max) != hipSuccess)) {
         (void)hipGetLastError();
@@ -959,6 +1001,12 @@ int scrna_plan_sync(scrna_plan* plan) {
     HIP_TRY(hipSetDevice(plan->ctx->device));
     HIP_TRY(hipEventSynchronize(plan->ev_stop));
     HIP_TRY(hipDeviceSynchronize());
+    for (const auto& lg : plan->groups) {
+        if (lg.kind != 0 || lg.nseg <= 1) continue;
+        int err = 0;
+        HIP_TRY(hipMemcpy(&err, plan->d_sched + lg.sched_off + SCHED_ERROR, sizeof(int), hipMemcpyDeviceToHost));
+        if (err) return set_error(SCRNA_ERR_DEVICE, "stepper segment waited past its time bound");
+    }
     return SCRNA_OK;
 }
 

@@ -26,6 +26,7 @@ struct scrna_ctx {
     size_t lds_max = 65536;          // per-workgroup LDS limit of the device
     size_t lds_limit_soft = 65536;   // keep double buffers under this to fit 2+ groups per CU
     int num_cu = 0;
+    int wall_khz = 100000;           // device wall-clock rate (wall_clock64 ticks per ms)
     int force_ch = 0;                // tile override (autotuner / tests); 0 = automatic
     hipStream_t stream = nullptr;
     std::mutex mu;
@@ -73,6 +74,8 @@ struct LaunchGroup {
     int mode = 0;                    // dense precision mode
     bool tele = false, inject = false;
     size_t lds = 0;
+    int nseg = 1;                    // FP32 stepper: segments per tile on whole-horizon launches
+    size_t sched_off = 0;            // its schedule block in scrna_plan::d_sched
 };
 
 }  // namespace scrna
@@ -96,6 +99,9 @@ struct scrna_plan {
     int32_t* d_counts = nullptr;
     uint16_t* d_wpool = nullptr;
     double* d_mom = nullptr;
+    int* d_sched = nullptr;               // segmented-schedule blocks of the FP32 stepper groups
+    size_t sched_words = 0;
+    long long wait_ticks = 0;             // bound on a segment's wait for its predecessor
     scrna::CaptureJob* d_cjobs = nullptr;
     scrna::MomentJob* d_mjobs_snap = nullptr;
     scrna::MomentJob* d_mjobs_state = nullptr;
@@ -132,7 +138,8 @@ int plan_launch(scrna_plan* P, hipStream_t st);
 int plan_download(scrna_plan* P, int32_t i, scrna_output* out);
 void plan_release(scrna_plan* P);
 int population_launch(scrna_plan* P, hipStream_t st);
-int launch_steps(scrna_plan* P, hipStream_t st, int s_begin, int s_end);
+int launch_steps(scrna_plan* P, hipStream_t st, int s_begin, int s_end, bool segmented);
+int choose_segments(int n_tiles, long long resident, int64_t steps);
 int plan_make_graph(scrna_plan* P);
 int unfused_prepare(scrna_plan* P, UnfusedNet& u);
 int unfused_launch(scrna_plan* P, UnfusedNet& u, hipStream_t st);

@@ -362,7 +362,7 @@ void worker_main(scrna_server* S) {
             const int nbuf = stepper_lds_bytes(Smax, 2) <= S->ctx->lds_limit_soft ? 2 : 1;
             e = launch_stepper(CH, NW, sp.second.second, S->blob_arena, S->d_tiles + sp.second.first,
                                (float*)S->arena, nullptr, (uint8_t*)S->arena, nullptr, nullptr, 0, INT32_MAX, nbuf,
-                               stepper_lds_bytes(Smax, nbuf), S->stream);
+                               stepper_lds_bytes(Smax, nbuf), nullptr, 1, 0, S->stream);
         }
         if (e == hipSuccess) e = hipStreamSynchronize(S->stream);
         if (e != hipSuccess) {
