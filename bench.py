@@ -13,8 +13,9 @@ timed region; outputs stay in HBM.  Prints one JSON line with
 
 Other workloads (--workload): g256 (precision comparison, 65,536 cells, 2000
 steps), batch (4096 uneven networks with population dynamics in one hipGraph),
-serve (Poisson arrivals at a fixed QPS), copy (plain device copy of known size,
-for scaling hardware byte counters).
+serve (Poisson arrivals at a fixed QPS), tauleap (binomial tau-leap on the G=20
+network), copy (plain device copy of known size, for scaling hardware byte
+counters).
 """
 from __future__ import annotations
 
@@ -193,7 +194,8 @@ def run_g256(a, sc) -> dict:
     out = {"workload": "G=256 additive network, 25% dense A, 8 AND + 8 OR, 65536 cells, 2000 steps dt=0.005",
            "cells": cells, "steps": steps, "results": {}}
     subsets = {}
-    for prec in ("fp32_unfused", "fp32", "bf16", "bf16x3"):
+    precs = [p for p in ("fp32_unfused", "fp32", "bf16", "bf16x3") if p in a.precisions.split(",")]
+    for prec in precs:
         plan = sc.Plan([net], sc.CLE(dt), kin, cell_num=cells, T=T, readout="both", seed=777, precision=prec)
         el, kms = timed_runs(plan, a.steps, a.warmup)
         cgs = float(net.G) * cells * steps
@@ -226,11 +228,53 @@ def run_g256(a, sc) -> dict:
                 errs[("mrna", "protein")[k]] = {"max_mean_rel_err": float(me.max()), "max_var_rel_err": float(ve.max())}
             out["results"][prec]["vs_fp64_replay"] = errs
         out["replay_cells"] = n_sub
-    base = out["results"]["fp32"]["cgs_per_s"]
-    base_ref = out["results"]["fp32_unfused"]["cgs_per_s"]
+    res = out["results"]
     for prec in ("fp32", "bf16", "bf16x3"):
-        out["results"][prec]["speedup_vs_fp32_fused"] = out["results"][prec]["cgs_per_s"] / base
-        out["results"][prec]["speedup_vs_fp32_reference_structured"] = out["results"][prec]["cgs_per_s"] / base_ref
+        if prec not in res:
+            continue
+        if "fp32" in res:
+            res[prec]["speedup_vs_fp32_fused"] = res[prec]["cgs_per_s"] / res["fp32"]["cgs_per_s"]
+        if "fp32_unfused" in res:
+            res[prec]["speedup_vs_fp32_reference_structured"] = res[prec]["cgs_per_s"] / res["fp32_unfused"]["cgs_per_s"]
+    return out
+
+
+def run_tauleap(a, sc) -> dict:
+    """GPU binomial tau-leap on the G=20 benchmark network: 100,000 cells, 5000
+    steps (dt=0.01), exact Poisson / binomial draws, FP64 propensities."""
+    from scrna_amd import samplers
+
+    wl = samplers.gpu_g20_workload()
+    cells = a.cells or wl["cells"]
+    net, kin = wl["net"], wl["kin"]
+    dt, T = wl["dt"], wl["T"]
+    plan = sc.Plan([net], sc.BinomialTauLeap(dt), kin, cell_num=cells, T=T, readout="both", seed=31)
+    steps = sc.n_steps(T, dt)
+    cgs = float(net.G) * cells * steps
+    el, kms = timed_runs(plan, a.steps, a.warmup)
+    y = plan.download(0).y
+    out = {"workload": f"binomial tau-leap, G=20 telegraph network, {cells} cells, {steps} steps dt={dt}",
+           "cgs_per_s": cgs * a.steps / el, "kernel_ms": float(np.mean(kms)), "dtype": "f64 propensities, i32 counts",
+           "mean_mrna": float(y[:, :net.G].mean()), "mean_protein": float(y[:, net.G:].mean())}
+    if not a.no_cpu:
+        from oracle import oracle as orc
+        threads = max(1, min(16, os.cpu_count() or 1))
+        probe = 4 * threads
+        t0 = time.perf_counter()
+        orc.tauleap(net, kin, dt=dt, n_steps=steps, cell_num=cells, cell_lo=0, cell_hi=probe, seed=31,
+                    threads=threads)
+        rate = probe / max(time.perf_counter() - t0, 1e-6)
+        sample = int(min(cells, max(probe, rate * 10.0)))
+        sample = max(threads, (sample // threads) * threads)
+        t0 = time.perf_counter()
+        orc.tauleap(net, kin, dt=dt, n_steps=steps, cell_num=cells, cell_lo=0, cell_hi=sample, seed=31,
+                    threads=threads)
+        el_cpu = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": float(net.G) * sample * steps / el_cpu, "unit": "cell*gene*steps/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"{sample} of {cells} cells, full horizon, FP64 C restatement of "
+                                         f"tauleap_binomial.jl, OpenMP over cells, {el_cpu:.1f} s"}
+    plan.close()
     return out
 
 
@@ -328,11 +372,12 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["g20", "g256", "batch", "serve", "copy"], default="g20")
+    ap.add_argument("--workload", choices=["g20", "g256", "batch", "serve", "copy", "tauleap"], default="g20")
     ap.add_argument("--cells", type=int, default=None, help="override cell count (testing only)")
     ap.add_argument("--tile", type=int, default=0, help="genes per lane for the FP32 stepper (2, 4, 8, 16)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--subset", type=int, default=256, help="g256: cells replayed by the FP64 oracle")
+    ap.add_argument("--precisions", default="fp32_unfused,fp32,bf16,bf16x3", help="g256: comma-separated forms")
     ap.add_argument("--networks", type=int, default=4096, help="batch: number of networks")
     ap.add_argument("--T", type=float, default=None, help="batch: horizon")
     ap.add_argument("--qps", type=float, default=4.0, help="serve: arrival rate")
@@ -352,7 +397,8 @@ def main() -> int:
 
     if a.tile:
         sc.default_context().set_tile(1, min(256, 16 * a.tile), a.tile)
-    runner = {"g20": run_g20, "g256": run_g256, "batch": run_batch, "serve": run_serve}[a.workload]
+    runner = {"g20": run_g20, "g256": run_g256, "batch": run_batch, "serve": run_serve,
+              "tauleap": run_tauleap}[a.workload]
     print(json.dumps(runner(a, sc)))
     return 0
 

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SCRNA_ABI_VERSION 1
+#define SCRNA_ABI_VERSION 2
 
 typedef enum scrna_status {
     SCRNA_OK = 0,
@@ -55,6 +55,15 @@ typedef enum scrna_precision {
     SCRNA_PREC_FP32_UNFUSED = 3 /* the reference GPU structure (ext:87-108): one kernel per
                                    broadcast, two SGEMMs per step, state in HBM -- a baseline */
 } scrna_precision;
+
+/* Simulation algorithm of a run (the reference dispatches on the algorithm
+ * type: simulate(net, alg::CLE, ...) ext:48-52, simulate(net,
+ * alg::BinomialTauLeap, ...) ext:48-117 / tauleap_binomial.jl:31-38). */
+typedef enum scrna_algorithm {
+    SCRNA_ALG_CLE = 0,                 /* Chemical Langevin Equation, Euler-Maruyama (cle.jl) */
+    SCRNA_ALG_BINOMIAL_TAU_LEAP = 1    /* binomial tau-leap (tauleap_binomial.jl), exact Poisson and
+                                          binomial draws, FP64 propensities, integer counts */
+} scrna_algorithm;
 
 /* GeneNetwork (network.jl:55-63) with CoopEdge / RedunEdge (network.jl:16-34).
  * Gate sources are listed in CSR form: gate e has sources
@@ -110,12 +119,15 @@ typedef struct scrna_run {
     double T;                     /* n_steps = ceil(T/dt) in double (cle.jl:56, ext:69) */
     int64_t cell_num;
     int32_t readout;              /* scrna_readout */
-    int32_t init_mode;            /* 0: ceil(steady state) (cle.jl:52-53); 1: zeros */
+    int32_t init_mode;            /* 0: steady state, ceil for the CLE (cle.jl:52-53), rounded half to
+                                     even for the tau-leap (tauleap_binomial.jl:59-60); 1: zeros */
     uint64_t seed;                /* Philox key */
     uint32_t net_id;              /* stream id of this network inside its request */
     int32_t precision;            /* scrna_precision */
     int64_t snapshot_every;       /* steps between snapshots; 0 = none */
     const float* injected_noise;  /* optional, host: [((s*G+g)*2+k)*cell_num + c], k=0 mRNA, 1 protein */
+    int32_t algorithm;            /* scrna_algorithm (0 = CLE) */
+    int32_t reserved0;            /* must be 0 */
 } scrna_run;
 
 /* Outputs of one network (all optional except y).

@@ -70,6 +70,10 @@ def load():
         lib.orc_cle_population.argtypes = [C.POINTER(OrcNetwork), C.POINTER(OrcKinetics), C.POINTER(OrcRun),
                                            C.POINTER(OrcPopulation), _dp, _dp, _dp, _up]
         lib.orc_cle_population.restype = C.c_int
+        lib.orc_tauleap_simulate.argtypes = lib.orc_cle_simulate.argtypes
+        lib.orc_tauleap_simulate.restype = C.c_int
+        lib.orc_poisson.argtypes = [C.c_double, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64]
+        lib.orc_poisson.restype = C.c_int64
         lib.orc_binomial.argtypes = [C.c_int64, C.c_double, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64]
         lib.orc_binomial.restype = C.c_int64
         lib.orc_capture.argtypes = [_dp, C.c_int64, C.c_int32, C.c_int64, C.c_double, C.c_double, C.c_uint64,
@@ -144,8 +148,21 @@ def regulation(net, kin, p, form="matrix"):
     return out
 
 
+def tauleap(net, kin, *, dt, T=None, n_steps=None, cell_num, cell_lo=0, cell_hi=None, seed=0, net_id=0,
+            init="steady", snapshot_every=0, threads=1):
+    """FP64 binomial tau-leap of cells [cell_lo, cell_hi) (tauleap_binomial.jl:31-180).
+    Same return layout as cle(); counts as float64."""
+    return cle(net, kin, dt=dt, T=T, n_steps=n_steps, cell_num=cell_num, cell_lo=cell_lo, cell_hi=cell_hi,
+               seed=seed, net_id=net_id, init=init, snapshot_every=snapshot_every, threads=threads,
+               _fn="orc_tauleap_simulate")
+
+
+def poisson(lam, c0, c1, c3, seed):
+    return int(load().orc_poisson(float(lam), c0, c1, c3, seed & 0xFFFFFFFFFFFFFFFF))
+
+
 def cle(net, kin, *, dt, T=None, n_steps=None, cell_num, cell_lo=0, cell_hi=None, seed=0, net_id=0,
-        init="steady", snapshot_every=0, injected=None, threads=1):
+        init="steady", snapshot_every=0, injected=None, threads=1, _fn="orc_cle_simulate"):
     """FP64 CLE of cells [cell_lo, cell_hi).  Returns dict(m, p, snaps_m, snaps_p, promoter)
     with arrays shaped (G, n_cells) / (n_snap, G, n_cells)."""
     lib = load()
@@ -168,7 +185,7 @@ def cle(net, kin, *, dt, T=None, n_steps=None, cell_num, cell_lo=0, cell_hi=None
     sm = np.empty((max(n_snap, 1), G, nc))
     sp = np.empty((max(n_snap, 1), G, nc))
     pr = np.empty((G, nc), dtype=np.uint8)
-    rc = lib.orc_cle_simulate(C.byref(s), C.byref(_kin(kin)), C.byref(run), m.ctypes.data_as(_dp),
+    rc = getattr(lib, _fn)(C.byref(s), C.byref(_kin(kin)), C.byref(run), m.ctypes.data_as(_dp),
                               p.ctypes.data_as(_dp), sm.ctypes.data_as(_dp) if n_snap else None,
                               sp.ctypes.data_as(_dp) if n_snap else None, pr.ctypes.data_as(_up))
     if rc:

@@ -50,7 +50,10 @@ enum {
     ORC_P_CAPTURE_BIN = 4, /* ctr = (cell, gene, block) */
     ORC_P_PARTITION = 5,   /* ctr = (mother, step, block), per (species,gene) sub-stream in word 3 bits */
     ORC_P_REPLACE = 6,     /* ctr = (mother, step, 0) */
-    ORC_P_VINIT = 7        /* ctr = (cell, 0, 0) */
+    ORC_P_VINIT = 7,       /* ctr = (cell, 0, 0) */
+    ORC_P_TAULEAP = 8      /* ctr = (cell, step, (gene << 16) + block): one 52-bit uniform stream per
+                              (cell, step, gene), read by transcription, mRNA decay, translation and
+                              protein decay in that order */
 };
 
 static inline void orc_draw(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
@@ -373,7 +376,9 @@ static double orc_fc(int64_t k) {
     return (1.0 / 12.0 - (1.0 / 360.0 - 1.0 / 1260.0 / kp2) / kp2) / kp;
 }
 
-static int64_t orc_binomial_stream(int64_t n, double p, orc_ustream* st) {
+/* lq: NAN, or log(1-p) and log(p) precomputed by the caller ({log1mp, logp});
+ * with them the inversion start q^n is exp(n * log q) instead of pow(q, n). */
+static int64_t orc_binomial_stream_lq(int64_t n, double p, const double* lq, orc_ustream* st) {
     if (n <= 0 || !(p > 0.0)) return 0;
     if (p >= 1.0) return n;
     int flip = p > 0.5;
@@ -382,7 +387,8 @@ static int64_t orc_binomial_stream(int64_t n, double p, orc_ustream* st) {
     int64_t x;
     if ((double)n * pp < 10.0) {
         /* inversion from 0 upwards (BINV) */
-        double s = pp / q, a = (double)(n + 1) * s, r0 = pow(q, (double)n);
+        double s = pp / q, a = (double)(n + 1) * s;
+        double r0 = lq ? exp((double)n * (flip ? lq[1] : lq[0])) : pow(q, (double)n);
         for (;;) {
             double u = orc_next_u(st), r = r0;
             int64_t k = 0;
@@ -448,6 +454,58 @@ static int64_t orc_binomial_stream(int64_t n, double p, orc_ustream* st) {
         }
     }
     return flip ? n - x : x;
+}
+
+static int64_t orc_binomial_stream(int64_t n, double p, orc_ustream* st) {
+    return orc_binomial_stream_lq(n, p, NULL, st);
+}
+
+/* log(k!) from the Stirling series and its correction table (orc_fc). */
+static double orc_logfact(int64_t k) {
+    double kd = (double)k;
+    return 0.91893853320467274178 + (kd + 0.5) * log(kd + 1.0) - (kd + 1.0) + orc_fc(k);
+}
+
+/* Poisson(lam): sequential inversion from 0 for lam < 10, otherwise PTRS,
+ * the transformed rejection of W. Hormann, "The transformed rejection method
+ * for generating Poisson random variables", Insurance: Mathematics and
+ * Economics 12 (1993) 39-45.  lam <= 0 (or NaN) draws nothing and returns 0. */
+static int64_t orc_poisson_stream(double lam, orc_ustream* st) {
+    if (!(lam > 0.0)) return 0;
+    if (lam < 10.0) {
+        const double p0 = exp(-lam);
+        for (;;) {
+            double u = orc_next_u(st), p = p0;
+            int64_t k = 0;
+            while (u > p) {
+                u -= p;
+                ++k;
+                if (k > 200) break;               /* u beyond the rounded total mass: redraw */
+                p *= lam / (double)k;
+            }
+            if (k <= 200) return k;
+        }
+    }
+    const double slam = sqrt(lam), loglam = log(lam);
+    const double b = 0.931 + 2.53 * slam;
+    const double a = -0.059 + 0.02483 * b;
+    const double alpha = 1.1239 + 1.1328 / (b - 3.4);
+    const double vr = 0.9277 - 3.6224 / (b - 2.0);
+    for (;;) {
+        double u = orc_next_u(st) - 0.5;
+        double v = orc_next_u(st);
+        double us = 0.5 - fabs(u);
+        double kf = floor((2.0 * a / us + b) * u + lam + 0.43);
+        if (us >= 0.07 && v <= vr) return (int64_t)kf;
+        if (kf < 0.0 || (us < 0.013 && v > us)) continue;
+        double w = v * alpha / (a / (us * us) + b);
+        if (log(w) <= -lam + kf * loglam - orc_logfact((int64_t)kf)) return (int64_t)kf;
+    }
+}
+
+int64_t orc_poisson(double lam, uint32_t ctr0, uint32_t ctr1, uint32_t c3, uint64_t seed) {
+    orc_ustream st = {seed, ctr0, ctr1, c3, 0, 2, {0, 0, 0, 0}};
+    return orc_poisson_stream(lam, &st);
 }
 
 int64_t orc_binomial(int64_t n, double p, uint32_t ctr0, uint32_t ctr1, uint32_t c3, uint64_t seed) {
@@ -649,4 +707,88 @@ void orc_moments(const double* m, const double* p, int64_t cells, int32_t G, int
         }
         for (int k = 0; k < 5; ++k) out[g * 5 + k] = s[k];
     }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Binomial tau-leap, independent cells                                     */
+/* (src/algorithms/tauleap_binomial.jl:31-180, regulation_mode :transcription) */
+/* ---------------------------------------------------------------------- */
+int orc_tauleap_simulate(const orc_network* net, const orc_kinetics* kin, const orc_run* run,
+                         double* m_out, double* p_out, double* snaps_m, double* snaps_p,
+                         uint8_t* promoter_out) {
+    if (orc_check(net, kin, run) || run->injected) return 1;
+    const int G = net->G;
+    const int64_t lo = run->cell_lo, hi = run->cell_hi, nc = hi - lo;
+    if (lo < 0 || hi > run->cell_num || nc < 0) return 1;
+    const double dt = run->dt;
+    const uint32_t c3 = (run->net_id << 4) | ORC_P_TAULEAP;
+    const int64_t n_snap = run->snapshot_every > 0 ? run->n_steps / run->snapshot_every : 0;
+    /* decay probabilities with dilution (tauleap_binomial.jl:145, 158) and their logs */
+    const double pdm = 1.0 - exp(-(kin->mu_m + kin->dilution) * dt);
+    const double pdp = 1.0 - exp(-(kin->mu_p + kin->dilution) * dt);
+    const double lqm[2] = {log(1.0 - pdm), log(pdm)};
+    const double lqp[2] = {log(1.0 - pdp), log(pdp)};
+    orc_tele tele;
+    orc_tele_init(net, dt, &tele);
+
+    /* init: round(Int, steady state), half to even (tauleap_binomial.jl:59-60) */
+    int64_t* m0 = (int64_t*)malloc(sizeof(int64_t) * (size_t)G);
+    int64_t* p0 = (int64_t*)malloc(sizeof(int64_t) * (size_t)G);
+    for (int g = 0; g < G; ++g) {
+        double mss = net->basals[g] / (kin->mu_m + kin->dilution);            /* kinetics.jl:65-70 */
+        double pss = kin->k_t * mss / (kin->mu_p + kin->dilution);
+        m0[g] = run->init_mode == 0 ? orc_round_even(mss) : 0;
+        p0[g] = run->init_mode == 0 ? orc_round_even(pss) : 0;
+    }
+
+    int nthreads = run->n_threads > 0 ? run->n_threads : 1;
+    (void)nthreads;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+    for (int64_t c = lo; c < hi; ++c) {
+        int64_t* M = (int64_t*)malloc(sizeof(int64_t) * (size_t)G * 2);
+        int64_t* P = M + G;
+        double* pf = (double*)malloc(sizeof(double) * (size_t)G * 2);
+        double* reg = pf + G;
+        uint8_t* on = (uint8_t*)malloc((size_t)G);
+        memcpy(M, m0, sizeof(int64_t) * (size_t)G);
+        memcpy(P, p0, sizeof(int64_t) * (size_t)G);
+        orc_promoter_init(&tele, G, run->seed, run->net_id << 4, (uint32_t)c, on);   /* :64-76 */
+        int64_t snap = 0;
+        for (int64_t s = 0; s < run->n_steps; ++s) {
+            for (int g = 0; g < G; ++g) pf[g] = (double)P[g];                     /* :80 */
+            if (tele.telegraph_any)                                               /* :83-123 */
+                orc_promoter_switch(&tele, G, run->seed, run->net_id << 4, (uint32_t)c, (uint32_t)s, on);
+            orc_regulation_loop(net, kin, pf, reg);                              /* :126 */
+            for (int g = 0; g < G; ++g) {
+                orc_ustream st = {run->seed, (uint32_t)c, (uint32_t)s, c3, (uint32_t)g << 16, 2, {0, 0, 0, 0}};
+                double prop = reg[g] + net->basals[g];                           /* :127 */
+                if (!(prop > 0.0)) prop = 0.0;
+                if (tele.telegraph_any && !on[g]) prop = 0.0;                    /* :134-137 */
+                int64_t n_tx = orc_poisson_stream(prop * dt, &st);                /* :140 */
+                int64_t n_md = M[g] > 0 ? orc_binomial_stream_lq(M[g], pdm, lqm, &st) : 0;   /* :143-149 */
+                M[g] += n_tx - n_md;                                             /* :151 */
+                int64_t mpos = M[g] > 0 ? M[g] : 0;
+                int64_t n_tl = orc_poisson_stream(kin->k_t * (double)mpos * dt, &st);     /* :154 */
+                int64_t n_pd = P[g] > 0 ? orc_binomial_stream_lq(P[g], pdp, lqp, &st) : 0;   /* :157-162 */
+                P[g] += n_tl - n_pd;                                             /* :164 */
+            }
+            if (n_snap > 0 && ((s + 1) % run->snapshot_every) == 0 && snap < n_snap) {
+                for (int g = 0; g < G; ++g) {
+                    size_t idx = ((size_t)snap * G + g) * (size_t)nc + (size_t)(c - lo);
+                    if (snaps_m) snaps_m[idx] = (double)M[g];
+                    if (snaps_p) snaps_p[idx] = (double)P[g];
+                }
+                ++snap;
+            }
+        }
+        for (int g = 0; g < G; ++g) {
+            if (m_out) m_out[(size_t)g * nc + (size_t)(c - lo)] = (double)M[g];
+            if (p_out) p_out[(size_t)g * nc + (size_t)(c - lo)] = (double)P[g];
+            if (promoter_out) promoter_out[(size_t)g * nc + (size_t)(c - lo)] = on[g];
+        }
+        free(M); free(pf); free(on);
+    }
+    free(m0); free(p0);
+    orc_tele_free(&tele);
+    return 0;
 }

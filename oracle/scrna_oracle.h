@@ -19,7 +19,10 @@
  *   - population CLE (cle.jl:143-284) with the binomial partitioning and
  *     Moran replacement of src/population.jl:157-205;
  *   - LogNormal-Binomial capture (src/capture.jl:25-47);
- *   - moment statistics (src/validate.jl:70-77).
+ *   - moment statistics (src/validate.jl:70-77);
+ *   - binomial tau-leap (src/algorithms/tauleap_binomial.jl:31-180, regulation
+ *     mode :transcription) with exact Poisson (inversion / PTRS) and binomial
+ *     (inversion / BTRD) draws.
  *
  * Random numbers: the reference draws from Julia's global or MersenneTwister
  * streams, which cannot be reproduced here.  This oracle and the GPU library
@@ -110,6 +113,18 @@ int orc_cle_simulate(const orc_network* net, const orc_kinetics* kin, const orc_
 int orc_cle_population(const orc_network* net, const orc_kinetics* kin, const orc_run* run,
                        const orc_population* pop, double* m_out, double* p_out,
                        double* V_out, uint8_t* promoter_out);
+
+/* Binomial tau-leap of cells [cell_lo, cell_hi) (tauleap_binomial.jl:31-180):
+ * same layouts as orc_cle_simulate; counts are returned as doubles.  init_mode
+ * 0 rounds the steady state half to even (tauleap_binomial.jl:59-60).
+ * injected must be NULL. */
+int orc_tauleap_simulate(const orc_network* net, const orc_kinetics* kin, const orc_run* run,
+                         double* m_out, double* p_out, double* snaps_m, double* snaps_p,
+                         uint8_t* promoter_out);
+
+/* Poisson(lam) drawn from the Philox stream with the given counter base
+ * (blocks counted in ctr[2]).  Exposed for tests. */
+int64_t orc_poisson(double lam, uint32_t ctr0, uint32_t ctr1, uint32_t c3, uint64_t seed);
 
 /* Binomial(n, p) drawn from the Philox stream with the given counter base
  * (ctr0, ctr1, key, c3); blocks are counted in ctr[2].  Exposed for tests. */

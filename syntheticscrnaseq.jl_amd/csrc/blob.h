@@ -60,7 +60,27 @@ enum BlobHeader : int {
     H_DENSE_LO,       // dense path: offset (bf16x8 units) of the packed W^T in the weight pool
     H_DENSE_HI,
     H_OFF_SLOTGENE,   // [S] i32 gene of each source slot
+    H_ALG,            // scrna_algorithm the network was compiled for
+    H_OFF_TL,         // tau-leap: FP64 section (TlWord head, then beta[G], edge weights[E],
+                      // gate weights by gate word [W], initial counts [G][2])
+    H_TL_NEDGES,      // tau-leap: E
+    H_TL_NGATEW,      // tau-leap: W (gate record words)
     H_WORDS = 64
+};
+
+// Doubles at the head of the tau-leap FP64 section.
+enum TlWord : int {
+    TL_KN = 0,        // K_d^n
+    TL_HILLN,         // n
+    TL_KT,            // k_t
+    TL_DT,            // dt
+    TL_PDM,           // 1 - exp(-(mu_m + dilution) dt), tauleap_binomial.jl:145
+    TL_LQM,           // log(1 - TL_PDM)
+    TL_LPM,           // log(TL_PDM)
+    TL_PDP,           // 1 - exp(-(mu_p + dilution) dt), tauleap_binomial.jl:158
+    TL_LQP,           // log(1 - TL_PDP)
+    TL_LPP,           // log(TL_PDP)
+    TL_HEAD = 16
 };
 
 enum BlobFlags : uint32_t {

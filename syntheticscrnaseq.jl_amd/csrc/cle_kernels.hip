@@ -23,6 +23,7 @@
 #include "device_binomial.h"
 #include "device_rng.h"
 #include "cle_kernels.h"
+#include "segment_schedule.h"
 
 namespace scrna {
 
@@ -79,17 +80,8 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
                    int nbuf, int* __restrict__ sched, int n_tiles, int n_seg, long long wait_ticks) {
     extern __shared__ float lds[];
     __shared__ int sh_ticket, sh_fail;
-    int ti = blockIdx.x, seg = 0;
-    if (sched != nullptr) {
-        // Segmented schedule: ticket t (taken in start order) is segment t / n_tiles
-        // of tile t % n_tiles, so the segment this one continues from took an
-        // earlier ticket and is already resident or finished.
-        if (threadIdx.x == 0) sh_ticket = atomicAdd(&sched[SCHED_TICKET], 1);
-        __syncthreads();
-        seg = sh_ticket / n_tiles;
-        ti = sh_ticket - seg * n_tiles;
-    }
-    const Tile tile = tiles[ti];
+    const SegmentSlot sl = segment_take(sched, n_tiles, &sh_ticket);
+    const Tile tile = tiles[sl.tile];
     const uint32_t* __restrict__ B = blobs + tile.blob;
     const int G = (int)B[H_G];
     const int n_steps = (int)B[H_NSTEPS];
@@ -97,34 +89,9 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
     // horizon (networks of a batch may differ in horizon), then this segment's share.
     const int r_lo = max(s_begin, tile.s0);
     const int r_hi = max(r_lo, min(min(s_end, tile.s1), n_steps));
-    const int s_start = r_lo + (int)(((long long)(r_hi - r_lo) * seg) / n_seg);
-    const int s_stop = r_lo + (int)(((long long)(r_hi - r_lo) * (seg + 1)) / n_seg);
-    if (sched != nullptr && seg > 0) {
-        // Wait for the previous segment of this tile (release/acquire at agent
-        // scope: the tiles' states cross XCD L2s).  The wait always ends because
-        // that segment holds an earlier ticket; the clock bound only turns a
-        // scheduling bug into a reported error instead of a hang.
-        if (threadIdx.x == 0) {
-            int fail = 0;
-            const long long t0 = wall_clock64();
-            while (__hip_atomic_load(&sched[SCHED_DONE + ti], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < seg) {
-                __builtin_amdgcn_s_sleep(16);
-                if (wall_clock64() - t0 > wait_ticks) {
-                    fail = 1;
-                    atomicOr(&sched[SCHED_ERROR], 1);
-                    break;
-                }
-            }
-            sh_fail = fail;
-        }
-        __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        if (sh_fail) {
-            if (threadIdx.x == 0) __hip_atomic_store(&sched[SCHED_DONE + ti], seg + 1, __ATOMIC_RELEASE,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
-            return;
-        }
-    }
+    int s_start, s_stop;
+    segment_range(r_lo, r_hi, sl.seg, n_seg, &s_start, &s_stop);
+    if (!segment_wait(sched, sl, wait_ticks, &sh_fail)) return;
     const bool resume = s_start > 0;       // later chunks and segments continue from the state pool
     const int S = (int)B[H_S];
     const uint32_t flags = B[H_FLAGS];
@@ -334,14 +301,7 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
         }
         if (pop && wave == 0) vol_pool[off_vol + cell] = V;
     }
-    if (sched != nullptr) {
-        // Publish this segment's state: every wave's stores complete at agent
-        // scope before the flag that releases the next segment.
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __syncthreads();
-        if (threadIdx.x == 0)
-            __hip_atomic_store(&sched[SCHED_DONE + ti], seg + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    segment_done(sched, sl);
 }
 
 // Resets one segmented-schedule block (ticket, error word, per-tile done

@@ -26,6 +26,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "blob.h"
 #include "cle_kernels.h"
 #include "device_rng.h"
@@ -174,7 +176,7 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
             // the quad draws the block of its element v = j and the quad rotates
             // words so every lane gets its gene's word for each of its cells.
             if (tele) {
-                const U4 u = philox4x32_10((uint32_t)cell(l & 3), (uint32_t)s, (uint32_t)(g[b] >> 2), c3s, k0, k1);
+                const U4 u = philox4x32_10_uk((uint32_t)cell(l & 3), (uint32_t)s, (uint32_t)(g[b] >> 2), c3s, k0, k1);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int j = l & 3;
@@ -189,25 +191,34 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
                     }
                 }
             }
-            // Hill values of the lane's gene for its 4 cells -> LDS.
+            // Hill values of the lane's gene for its 4 cells -> LDS.  The n = 2
+            // test sits outside the cell loop so that the n = 2 path never
+            // evaluates log/exp (as a select, both sides would run).
+            auto hill_phase = [&](auto h2) {
+                constexpr bool H2 = decltype(h2)::value;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                float act = 0.0f, rep = 1.0f;
-                if (gl[b]) {
-                    const float x = p[b][v] * invK;
-                    float xn = hill2 ? x * x : __builtin_amdgcn_exp2f(hill_n * __builtin_amdgcn_logf(x));
-                    xn = fminf(xn, 1e30f);
-                    rep = __builtin_amdgcn_rcpf(1.0f + xn);
-                    act = xn * rep;
+                for (int v = 0; v < 4; ++v) {
+                    float act = 0.0f, rep = 1.0f;
+                    if (gl[b]) {
+                        const float x = p[b][v] * invK;
+                        float xn = H2 ? x * x : __builtin_amdgcn_exp2f(hill_n * __builtin_amdgcn_logf(x));
+                        xn = fminf(xn, 1e30f);
+                        rep = __builtin_amdgcn_rcpf(1.0f + xn);
+                        act = xn * rep;
+                    }
+                    const __bf16 hi = (__bf16)act;
+                    T0[row(v) * rowstride + g[b]] = hi;
+                    if (MODE == 2) T1[row(v) * rowstride + g[b]] = (__bf16)(act - (float)hi);
+                    if (slot[b] >= 0) {
+                        GH[(2 * slot[b]) * 16 + row(v)] = act;
+                        GH[(2 * slot[b] + 1) * 16 + row(v)] = rep;
+                    }
                 }
-                const __bf16 hi = (__bf16)act;
-                T0[row(v) * rowstride + g[b]] = hi;
-                if (MODE == 2) T1[row(v) * rowstride + g[b]] = (__bf16)(act - (float)hi);
-                if (slot[b] >= 0) {
-                    GH[(2 * slot[b]) * 16 + row(v)] = act;
-                    GH[(2 * slot[b] + 1) * 16 + row(v)] = rep;
-                }
-            }
+            };
+            if (hill2)
+                hill_phase(std::true_type{});
+            else
+                hill_phase(std::false_type{});
         }
         __syncthreads();
 
@@ -276,7 +287,7 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
 #pragma unroll
                 for (int t = 0; t < 2; ++t) {
                     const int cm = par ? cell(t + 2) : cell(t);
-                    const U4 u = philox4x32_10((uint32_t)cm, (uint32_t)s, (uint32_t)(g[b] >> 1), c3n, k0, k1);
+                    const U4 u = philox4x32_10_uk((uint32_t)cm, (uint32_t)s, (uint32_t)(g[b] >> 1), c3n, k0, k1);
                     const uint32_t mine_a = par ? u.z : u.x, mine_b = par ? u.w : u.y;
                     const uint32_t got_a = (uint32_t)__shfl_xor((int)(par ? u.x : u.z), 1);
                     const uint32_t got_b = (uint32_t)__shfl_xor((int)(par ? u.y : u.w), 1);

@@ -54,7 +54,10 @@ __device__ __forceinline__ double stirling_fc(int64_t k) {
     return (1.0 / 12.0 - (1.0 / 360.0 - 1.0 / 1260.0 / kp2) / kp2) / kp;
 }
 
-__device__ inline int64_t binomial_sample(int64_t n, double p, UStream& st) {
+// lq: nullptr, or {log(1-p), log(p)} precomputed by the caller, in which case
+// the inversion start q^n is exp(n * log q) instead of pow(q, n) (a fixed
+// decay probability drawn every step).
+__device__ inline int64_t binomial_sample(int64_t n, double p, UStream& st, const double* lq = nullptr) {
 #pragma clang fp contract(off)
     if (n <= 0 || !(p > 0.0)) return 0;
     if (p >= 1.0) return n;
@@ -63,7 +66,8 @@ __device__ inline int64_t binomial_sample(int64_t n, double p, UStream& st) {
     const double q = 1.0 - pp;
     int64_t x = 0;
     if ((double)n * pp < 10.0) {
-        const double s = pp / q, a = (double)(n + 1) * s, r0 = pow(q, (double)n);
+        const double s = pp / q, a = (double)(n + 1) * s;
+        const double r0 = lq ? exp((double)n * (flip ? lq[1] : lq[0])) : pow(q, (double)n);
         for (;;) {
             double u = st.next(), r = r0;
             int64_t k = 0;
@@ -134,6 +138,51 @@ __device__ inline int64_t binomial_sample(int64_t n, double p, UStream& st) {
         }
     }
     return flip ? n - x : x;
+}
+
+// log(k!) from the Stirling series and its correction table.
+__device__ __forceinline__ double log_factorial(int64_t k) {
+#pragma clang fp contract(off)
+    const double kd = (double)k;
+    return 0.91893853320467274178 + (kd + 0.5) * log(kd + 1.0) - (kd + 1.0) + stirling_fc(k);
+}
+
+// Exact Poisson(lam): sequential inversion from 0 for lam < 10, otherwise
+// PTRS (W. Hormann, "The transformed rejection method for generating Poisson
+// random variables", Insurance: Mathematics and Economics 12, 1993).  lam <= 0
+// draws nothing and returns 0.
+__device__ inline int64_t poisson_sample(double lam, UStream& st) {
+#pragma clang fp contract(off)
+    if (!(lam > 0.0)) return 0;
+    if (lam < 10.0) {
+        const double p0 = exp(-lam);
+        for (;;) {
+            double u = st.next(), p = p0;
+            int64_t k = 0;
+            while (u > p) {
+                u -= p;
+                ++k;
+                if (k > 200) break;   // u beyond the rounded total mass: redraw
+                p *= lam / (double)k;
+            }
+            if (k <= 200) return k;
+        }
+    }
+    const double slam = sqrt(lam), loglam = log(lam);
+    const double b = 0.931 + 2.53 * slam;
+    const double a = -0.059 + 0.02483 * b;
+    const double alpha = 1.1239 + 1.1328 / (b - 3.4);
+    const double vr = 0.9277 - 3.6224 / (b - 2.0);
+    for (;;) {
+        const double u = st.next() - 0.5;
+        const double v = st.next();
+        const double us = 0.5 - fabs(u);
+        const double kf = floor((2.0 * a / us + b) * u + lam + 0.43);
+        if (us >= 0.07 && v <= vr) return (int64_t)kf;
+        if (kf < 0.0 || (us < 0.013 && v > us)) continue;
+        const double w = v * alpha / (a / (us * us) + b);
+        if (log(w) <= -lam + kf * loglam - log_factorial((int64_t)kf)) return (int64_t)kf;
+    }
 }
 
 }  // namespace scrna

@@ -21,7 +21,8 @@ enum Purpose : uint32_t {
     kCaptureBin = 4,   // (cell, gene, block)
     kPartition = 5,    // (mother, step, ((2*gene+species)<<16) + block)
     kReplace = 6,      // (mother, step, 0)
-    kVInit = 7         // (cell, 0, 0)
+    kVInit = 7,        // (cell, 0, 0)
+    kTauLeap = 8       // (cell, step, (gene << 16) + block): the tau-leap draws of one gene
 };
 
 struct U4 {

@@ -166,6 +166,17 @@ int validate_inputs(const scrna_network* net, const scrna_kinetics* kin, const s
     if (run->init_mode < 0 || run->init_mode > 1) return set_error(SCRNA_ERR_INVALID, "init_mode must be 0 or 1");
     if (run->snapshot_every < 0) return set_error(SCRNA_ERR_INVALID, "snapshot_every must be >= 0");
     if (run->precision < 0 || run->precision > 3) return set_error(SCRNA_ERR_INVALID, "unknown precision %d", run->precision);
+    if (run->algorithm != SCRNA_ALG_CLE && run->algorithm != SCRNA_ALG_BINOMIAL_TAU_LEAP)
+        return set_error(SCRNA_ERR_INVALID, "unknown algorithm %d", run->algorithm);
+    if (run->reserved0 != 0) return set_error(SCRNA_ERR_INVALID, "scrna_run.reserved0 must be 0");
+    if (run->algorithm == SCRNA_ALG_BINOMIAL_TAU_LEAP) {
+        if (run->precision != SCRNA_PREC_FP32)
+            return set_error(SCRNA_ERR_UNSUPPORTED, "the tau-leap computes propensities in FP64; precision must be 0");
+        if (run->injected_noise)
+            return set_error(SCRNA_ERR_INVALID, "injected noise applies to the CLE only");
+        if (pop && pop->enabled)
+            return set_error(SCRNA_ERR_UNSUPPORTED, "population dynamics are implemented for the CLE only");
+    }
     if (run->precision == SCRNA_PREC_FP32_UNFUSED) {
         bool tele = false;
         if (net->k_on)
@@ -222,6 +233,13 @@ int compile_network(const scrna_network* net, const scrna_kinetics* kin, const s
         CH = force_ch;
         NW = (G + CH - 1) / CH;
         if (NW > 16) return set_error(SCRNA_ERR_INVALID, "tile override CH=%d gives %d waves > 16", CH, NW);
+    }
+    if (run->algorithm == SCRNA_ALG_BINOMIAL_TAU_LEAP) {
+        // Tau-leap: genes spread over at most 8 waves (the FP64 samplers need
+        // the register budget of 2 waves per SIMD); genes per wave is a run-time value.
+        NW = std::min(tauleap_max_waves(), G);
+        CH = (G + NW - 1) / NW;
+        NW = (G + CH - 1) / CH;
     }
     out->CH = CH;
     out->NW = NW;
@@ -372,8 +390,12 @@ int compile_network(const scrna_network* net, const scrna_kinetics* kin, const s
     }
     align16();
     b[H_OFF_GATES] = (uint32_t)b.size();
+    std::vector<double> gate_w;   // FP64 strength*beta by gate word (tau-leap)
     for (int g = 0; g < G; ++g)
         for (auto& r : gates_of[g]) {
+            gate_w.push_back(r.strength_beta);
+            gate_w.push_back(0.0);
+            for (size_t i = 0; i < r.src.size(); ++i) gate_w.push_back(0.0);
             b.push_back((r.type << 24) | (uint32_t)r.src.size());
             b.push_back(f2w((float)r.strength_beta));
             for (int s : r.src) b.push_back((uint32_t)slot_of[s]);
@@ -419,6 +441,45 @@ int compile_network(const scrna_network* net, const scrna_kinetics* kin, const s
     for (int g = 0; g < G; ++g)
         if (slot_of[g] >= 0) b.push_back((uint32_t)g);
     align16();
+    b[H_ALG] = (uint32_t)run->algorithm;
+    out->algorithm = run->algorithm;
+    if (run->algorithm == SCRNA_ALG_BINOMIAL_TAU_LEAP) {
+        // FP64 section of the tau-leap: the values the CPU restatement uses,
+        // computed with the same expressions (tauleap_binomial.jl:59-60, 145, 158).
+        std::vector<double> d(TL_HEAD, 0.0);
+        const double pdm = 1.0 - std::exp(-(kin->mu_m + kin->dilution) * run->dt);
+        const double pdp = 1.0 - std::exp(-(kin->mu_p + kin->dilution) * run->dt);
+        d[TL_KN] = std::pow(kin->K_d, kin->n);
+        d[TL_HILLN] = kin->n;
+        d[TL_KT] = kin->k_t;
+        d[TL_DT] = run->dt;
+        d[TL_PDM] = pdm;
+        d[TL_LQM] = std::log(1.0 - pdm);
+        d[TL_LPM] = std::log(pdm);
+        d[TL_PDP] = pdp;
+        d[TL_LQP] = std::log(1.0 - pdp);
+        d[TL_LPP] = std::log(pdp);
+        for (int g = 0; g < G; ++g) d.push_back(net->basals[g]);
+        for (int g = 0; g < G; ++g)
+            for (auto& ed : in_edges[g]) d.push_back(std::fabs(ed.second));
+        d.insert(d.end(), gate_w.begin(), gate_w.end());
+        for (int g = 0; g < G; ++g) {
+            const double mss = net->basals[g] / (kin->mu_m + kin->dilution);   // kinetics.jl:67
+            const double pss = kin->k_t * mss / (kin->mu_p + kin->dilution);   // kinetics.jl:68
+            d.push_back(run->init_mode == 0 ? std::nearbyint(mss) : 0.0);      // round half to even
+            d.push_back(run->init_mode == 0 ? std::nearbyint(pss) : 0.0);
+        }
+        b[H_TL_NEDGES] = (uint32_t)out->n_edges;
+        b[H_TL_NGATEW] = (uint32_t)gate_w.size();
+        b[H_OFF_TL] = (uint32_t)b.size();   // 16-word aligned, so the doubles are 8-byte aligned
+        for (double v : d) {
+            uint64_t u;
+            std::memcpy(&u, &v, 8);
+            b.push_back((uint32_t)u);
+            b.push_back((uint32_t)(u >> 32));
+        }
+        align16();
+    }
     b[H_GP] = (uint32_t)out->Gp;
     b[H_OFF_BIAS] = (uint32_t)b.size();
     for (int g = 0; g < G; ++g) b.push_back(f2w(dense ? (float)bias[g] : 0.0f));
@@ -569,7 +630,8 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
             continue;
         }
         const bool tele = (cn.blob[H_FLAGS] & F_TELE) != 0, inj = (cn.blob[H_FLAGS] & F_INJECT) != 0;
-        GKey key = cn.dense ? GKey{1, cn.mode, cn.NW, tele ? 1 : 0, inj ? 1 : 0} : GKey{0, cn.CH, cn.NW, 0, 0};
+        GKey key = cn.dense ? GKey{1, cn.mode, cn.NW, tele ? 1 : 0, inj ? 1 : 0}
+                            : GKey{cn.algorithm == SCRNA_ALG_BINOMIAL_TAU_LEAP ? 2 : 0, cn.CH, cn.NW, 0, 0};
         auto& v = groups[key];
         auto& sg = group_SG[key];
         sg.first = std::max(sg.first, cn.S);
@@ -603,12 +665,15 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
         lg.tile0 = (int)all_tiles.size();
         lg.n_tiles = (int)kv.second.size();
         const int S = std::max(1, group_SG[kv.first].first);
-        if (lg.kind == 0) {
+        if (lg.kind == 0 || lg.kind == 2) {
+            const bool tl = lg.kind == 2;
             lg.CH = kv.first.a;
-            lg.nbuf = stepper_lds_bytes(S, 2) <= ctx->lds_limit_soft ? 2 : 1;
-            lg.lds = stepper_lds_bytes(S, lg.nbuf);
-            const long long resident = (long long)stepper_blocks_per_cu(lg.CH, lg.NW, lg.lds) * ctx->num_cu;
-            lg.nseg = choose_segments(lg.n_tiles, resident, group_steps[kv.first]);
+            const size_t two = tl ? tauleap_lds_bytes(S, 2, lg.NW, lg.CH) : stepper_lds_bytes(S, 2);
+            lg.nbuf = two <= ctx->lds_limit_soft ? 2 : 1;
+            lg.lds = tl ? tauleap_lds_bytes(S, lg.nbuf, lg.NW, lg.CH) : stepper_lds_bytes(S, lg.nbuf);
+            const int per_cu = tl ? tauleap_blocks_per_cu(lg.NW, lg.lds)
+                                  : stepper_blocks_per_cu(lg.CH, lg.NW, lg.lds);
+            lg.nseg = choose_segments(lg.n_tiles, (long long)per_cu * ctx->num_cu, group_steps[kv.first]);
             if (lg.nseg > 1) {
                 lg.sched_off = P->sched_words;
                 P->sched_words += sched_words(lg.n_tiles);
@@ -620,7 +685,7 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
             lg.nbuf = 2;
             lg.lds = dense_lds_bytes(group_SG[kv.first].second, S, lg.mode);
         }
-        if (lg.lds + (lg.kind == 0 ? kStepperStaticLds : 0) > ctx->lds_max)
+        if (lg.lds + (lg.kind != 1 ? kStepperStaticLds : 0) > ctx->lds_max)
             return set_error(SCRNA_ERR_UNSUPPORTED, "%d source genes need %zu B of LDS per workgroup (max %zu)", S,
                              lg.lds, ctx->lds_max);
         all_tiles.insert(all_tiles.end(), kv.second.begin(), kv.second.end());
@@ -784,10 +849,10 @@ int choose_segments(int n_tiles, long long resident, int64_t steps) {
 
 // Run steps [s_begin, s_end) of every network (clamped to each horizon).
 // A tile resumes from the state pool exactly when its first step is > 0.
-// `segmented` enables the segmented schedule of the FP32 stepper groups.
+// `segmented` enables the segmented schedule of the FP32 and tau-leap stepper groups.
 int launch_steps(scrna_plan* P, hipStream_t st, int s_begin, int s_end, bool segmented) {
     for (const LaunchGroup& lg : P->groups) {
-        if (lg.kind == 0) {
+        if (lg.kind == 0 || lg.kind == 2) {
             int* sched = nullptr;
             int nseg = 1;
             if (segmented && lg.nseg > 1) {
@@ -795,9 +860,14 @@ int launch_steps(scrna_plan* P, hipStream_t st, int s_begin, int s_end, bool seg
                 nseg = lg.nseg;
                 HIP_TRY(launch_sched_reset(sched, (int)sched_words(lg.n_tiles), st));
             }
-            HIP_TRY(launch_stepper(lg.CH, lg.NW, lg.n_tiles, P->d_blobs, P->d_tiles + lg.tile0, P->d_state,
-                                   P->d_snap, P->d_prom, P->d_vol, P->d_inj, s_begin, s_end, lg.nbuf, lg.lds, sched,
-                                   nseg, P->wait_ticks, st));
+            if (lg.kind == 0)
+                HIP_TRY(launch_stepper(lg.CH, lg.NW, lg.n_tiles, P->d_blobs, P->d_tiles + lg.tile0, P->d_state,
+                                       P->d_snap, P->d_prom, P->d_vol, P->d_inj, s_begin, s_end, lg.nbuf, lg.lds,
+                                       sched, nseg, P->wait_ticks, st));
+            else
+                HIP_TRY(launch_tauleap(lg.CH, lg.NW, lg.n_tiles, P->d_blobs, P->d_tiles + lg.tile0, P->d_state,
+                                       P->d_snap, P->d_prom, s_begin, s_end, lg.nbuf, lg.lds, sched, nseg,
+                                       P->wait_ticks, st));
         } else
             HIP_TRY(launch_dense(lg.mode, lg.tele, lg.inject, lg.NW, lg.n_tiles, P->d_blobs, P->d_tiles + lg.tile0,
                                  P->d_wpool, P->d_state, P->d_snap, P->d_prom, P->d_inj, s_begin, s_end, lg.lds,
@@ -960,7 +1030,8 @@ int scrna_ctx_create(int32_t device, uint64_t hbm_budget_bytes, scrna_ctx** out)
     if (hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && wall_khz > 0)
         C->wall_khz = wall_khz;
     if (C->lds_max > 65536 && (set_stepper_lds_limit(C->lds_max) != hipSuccess ||
-                               set_dense_lds_limit(C->lds_max) != hipSuccess)) {
+                               set_dense_lds_limit(C->lds_max) != hipSuccess ||
+                               set_tauleap_lds_limit(C->lds_max) != hipSuccess)) {
         (void)hipGetLastError();
         C->lds_max = 65536;   // large dynamic LDS not granted: stay within the default limit
     }
@@ -1002,7 +1073,7 @@ int scrna_plan_sync(scrna_plan* plan) {
     HIP_TRY(hipEventSynchronize(plan->ev_stop));
     HIP_TRY(hipDeviceSynchronize());
     for (const auto& lg : plan->groups) {
-        if (lg.kind != 0 || lg.nseg <= 1) continue;
+        if (lg.kind == 1 || lg.nseg <= 1) continue;
         int err = 0;
         HIP_TRY(hipMemcpy(&err, plan->d_sched + lg.sched_off + SCHED_ERROR, sizeof(int), hipMemcpyDeviceToHost));
         if (err) return set_error(SCRNA_ERR_DEVICE, "stepper segment waited past its time bound");

@@ -42,7 +42,7 @@ struct CompiledNet {
     size_t blob_off = 0;
     int G = 0, S = 0, CH = 2, NW = 1, n_edges = 0;
     int64_t cells = 0, n_steps = 0, snap_every = 0, n_snap = 0;
-    int readout = 1, precision = 0;
+    int readout = 1, precision = 0, algorithm = 0;
     bool capture = false, population = false;
     scrna_capture cap{};
     scrna_population pop{};
@@ -69,7 +69,7 @@ struct UnfusedNet {
 };
 
 struct LaunchGroup {
-    int kind = 0;                    // 0: sparse VALU stepper, 1: dense MFMA stepper
+    int kind = 0;                    // 0: sparse VALU stepper, 1: dense MFMA stepper, 2: tau-leap stepper
     int CH = 2, NW = 1, tile0 = 0, n_tiles = 0, nbuf = 1;
     int mode = 0;                    // dense precision mode
     bool tele = false, inject = false;

@@ -438,8 +438,10 @@ int scrna_submit(scrna_server* S, const scrna_request* req, int64_t* ticket) {
     const scrna_capture* cap = req->capture.enabled ? &req->capture : nullptr;
     int rc = validate_inputs(&req->net, &req->kin, &req->run, nullptr, cap);
     if (rc) return rc;
-    if (req->run.precision != SCRNA_PREC_FP32 || req->run.snapshot_every != 0 || req->run.injected_noise)
-        return set_error(SCRNA_ERR_UNSUPPORTED, "served requests run FP32 regulation without snapshots or injected noise");
+    if (req->run.precision != SCRNA_PREC_FP32 || req->run.snapshot_every != 0 || req->run.injected_noise ||
+        req->run.algorithm != SCRNA_ALG_CLE)
+        return set_error(SCRNA_ERR_UNSUPPORTED,
+                         "served requests run the FP32 CLE without snapshots or injected noise");
     std::unique_ptr<Req> r(new Req());
     rc = compile_network(&req->net, &req->kin, &req->run, nullptr, 0, &r->cn);
     if (rc) return rc;

@@ -8,7 +8,7 @@ gfx950 device; nothing here computes a trajectory on the CPU.
 """
 from . import analytical, samplers
 from ._lib import BudgetError, Context, ScrnaError, UnsupportedError, default_context, load
-from .cle import CLE, Plan, SimResult, apply_capture, n_steps, simulate, simulate_gpu_batch
+from .cle import CLE, BinomialTauLeap, Plan, SimResult, apply_capture, n_steps, simulate, simulate_gpu_batch
 from .kinetics import CaptureModel, KineticParams, PopulationConfig, steady_state
 from .network import (CoopEdge, GeneNetwork, RedunEdge, gene_network_from_theta, hill_activation,
                       hill_repression, is_bursty, n_species, precompute_hill_matrices)
@@ -16,7 +16,7 @@ from .serving import Server
 
 __all__ = [
     "BudgetError", "Context", "ScrnaError", "UnsupportedError", "default_context", "load",
-    "CLE", "Plan", "SimResult", "apply_capture", "n_steps", "simulate", "simulate_gpu_batch",
+    "CLE", "BinomialTauLeap", "Plan", "SimResult", "apply_capture", "n_steps", "simulate", "simulate_gpu_batch",
     "CaptureModel", "KineticParams", "PopulationConfig", "steady_state",
     "CoopEdge", "GeneNetwork", "RedunEdge", "gene_network_from_theta", "hill_activation",
     "hill_repression", "is_bursty", "n_species", "precompute_hill_matrices",

@@ -87,6 +87,8 @@ class ScrnaRun(C.Structure):
         ("precision", C.c_int32),
         ("snapshot_every", C.c_int64),
         ("injected_noise", c_float_p),
+        ("algorithm", C.c_int32),
+        ("reserved0", C.c_int32),
     ]
 
 

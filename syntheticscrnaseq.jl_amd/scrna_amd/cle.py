@@ -1,9 +1,11 @@
-"""GPU CLE entry points (the drop-in for ext/SyntheticscRNAseqCUDAExt.jl).
+"""GPU simulation entry points (the drop-in for ext/SyntheticscRNAseqCUDAExt.jl).
 
 ``simulate(net, CLE(dt), kin, "gpu"; cell_num, T, readout)`` replaces the
 CUDA.jl method at ext:48-117 (and ext:409-529 with a PopulationConfig);
-``simulate_gpu_batch`` replaces ext:132-213; ``apply_capture`` is the GPU form
-of src/capture.jl:25-47.  All work runs in libscrna.so's HIP kernels.
+``simulate(net, BinomialTauLeap(dt), kin, "gpu"; ...)`` replaces ext:230-311;
+``simulate_gpu_batch`` replaces ext:132-213 and ext:317-399; ``apply_capture``
+is the GPU form of src/capture.jl:25-47.  All work runs in libscrna.so's HIP
+kernels.
 """
 from __future__ import annotations
 
@@ -23,6 +25,23 @@ from .network import GeneNetwork
 class CLE:
     """Euler-Maruyama CLE with collapsed noise (cle.jl:21-23)."""
     dt: float
+
+
+@dataclass(frozen=True)
+class BinomialTauLeap:
+    """Binomial tau-leap (tauleap_binomial.jl:19-22): Poisson creation and
+    binomial decay per step, integer counts.  The GPU draws are exact (the
+    reference GPU form truncates a Poisson inverse CDF and approximates the
+    binomial decay by a clamped Poisson, ext:21-39)."""
+    dt: float
+
+
+def _algorithm(alg) -> int:
+    if isinstance(alg, CLE):
+        return 0
+    if isinstance(alg, BinomialTauLeap):
+        return 1
+    raise TypeError("the GPU path implements CLE and BinomialTauLeap")
 
 
 @dataclass
@@ -45,12 +64,13 @@ def _seed(seed) -> int:
     return int(seed) & 0xFFFFFFFFFFFFFFFF
 
 
-def _run(alg: CLE, cell_num: int, T: float, readout: str, init: str, seed, net_id: int,
+def _run(alg, cell_num: int, T: float, readout: str, init: str, seed, net_id: int,
          precision: str, snapshot_every: int, injected) -> tuple:
     if readout not in _lib.READOUT:
         raise ValueError("readout must be 'mrna', 'protein' or 'both'")
     if init not in ("steady", "zeros"):
-        raise ValueError("init must be 'steady' (ceil of the steady state, cle.jl:52-53) or 'zeros'")
+        raise ValueError("init must be 'steady' (the steady state, cle.jl:52-53 / tauleap_binomial.jl:59-60) "
+                         "or 'zeros'")
     if precision not in _lib.PRECISION:
         raise ValueError("precision must be 'fp32', 'bf16' or 'bf16x3'")
     keep = []
@@ -61,7 +81,7 @@ def _run(alg: CLE, cell_num: int, T: float, readout: str, init: str, seed, net_i
         inj_p = inj.ctypes.data_as(_lib.c_float_p)
     r = _lib.ScrnaRun(float(alg.dt), float(T), int(cell_num), _lib.READOUT[readout],
                       0 if init == "steady" else 1, _seed(seed), int(net_id), _lib.PRECISION[precision],
-                      int(snapshot_every), inj_p)
+                      int(snapshot_every), inj_p, _algorithm(alg), 0)
     return r, keep
 
 
@@ -115,7 +135,7 @@ def _listify(x, n):
     return [x] * n
 
 
-def simulate_gpu_batch(networks: Sequence[GeneNetwork], alg: CLE, kin, *, cell_num=1000, T=50.0,
+def simulate_gpu_batch(networks: Sequence[GeneNetwork], alg, kin, *, cell_num=1000, T=50.0,
                        readout: str = "protein", population=None, capture=None, seed=None,
                        init: str = "steady", snapshot_every: int = 0, precision: str = "fp32",
                        injected_noise=None, full: bool = False, ctx: _lib.Context | None = None):
@@ -130,8 +150,7 @@ def simulate_gpu_batch(networks: Sequence[GeneNetwork], alg: CLE, kin, *, cell_n
     n = len(nets)
     if n == 0:
         raise ValueError("no networks")
-    if not isinstance(alg, CLE):
-        raise TypeError("the GPU path implements the CLE algorithm only")
+    _algorithm(alg)
     ctx = ctx or _lib.default_context()
     kins = _listify(kin, n)
     cells = _listify(cell_num, n)
@@ -162,7 +181,7 @@ def simulate_gpu_batch(networks: Sequence[GeneNetwork], alg: CLE, kin, *, cell_n
     return results if full else [r.y for r in results]
 
 
-def simulate(net: GeneNetwork, alg: CLE, kin: KineticParams, backend: str = "gpu",
+def simulate(net: GeneNetwork, alg, kin: KineticParams, backend: str = "gpu",
              population: PopulationConfig | None = None, *, cell_num: int = 1000, T: float = 50.0,
              readout: str = "protein", capture: CaptureModel | None = None, seed=None,
              init: str = "steady", snapshot_every: int = 0, precision: str = "fp32",
@@ -209,7 +228,7 @@ def apply_capture(X, capture: CaptureModel, *, seed=None, net_id: int = 0,
 class Plan:
     """Device-resident batch: upload once, launch many times (scrna_plan_*)."""
 
-    def __init__(self, networks, alg: CLE, kin, *, cell_num=1000, T=50.0, readout="protein",
+    def __init__(self, networks, alg, kin, *, cell_num=1000, T=50.0, readout="protein",
                  population=None, capture=None, seed=None, init="steady", snapshot_every=0,
                  precision="fp32", ctx: _lib.Context | None = None):
         nets = list(networks)

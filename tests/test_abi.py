@@ -34,7 +34,7 @@ def test_ctypes_signatures_cover_header():
 
 def test_version_calls_without_device():
     lib = _lib.load()
-    assert lib.scrna_abi_version() == 1
+    assert lib.scrna_abi_version() == 2
     assert b"gfx950" in lib.scrna_version()
 
 
@@ -47,7 +47,7 @@ STRUCTS = {
                                                 "div_check_interval", "partition", "replace_mode"]),
     "scrna_capture": (_lib.ScrnaCapture, ["enabled", "efficiency", "efficiency_std", "readout"]),
     "scrna_run": (_lib.ScrnaRun, ["dt", "T", "cell_num", "readout", "init_mode", "seed", "net_id", "precision",
-                                  "snapshot_every", "injected_noise"]),
+                                  "snapshot_every", "injected_noise", "algorithm", "reserved0"]),
     "scrna_output": (_lib.ScrnaOutput, ["y", "ld", "counts", "snapshots", "moments", "volumes"]),
     "scrna_request": (_lib.ScrnaRequest, ["net", "kin", "run", "capture"]),
     "scrna_latency": (_lib.ScrnaLatency, ["p50_ms", "p99_ms", "mean_ms", "max_ms", "cell_gene_steps", "busy_s",
