@@ -120,7 +120,7 @@ typedef struct scrna_run {
     int64_t cell_num;
     int32_t readout;              /* scrna_readout */
     int32_t init_mode;            /* 0: steady state, ceil for the CLE (cle.jl:52-53), rounded half to
-                                     even for the tau-leap (tauleap_binomial.jl:59-60); 1: zeros */
+                                     even for the tau-leap (tauleap_binomial.jl:61-62); 1: zeros */
     uint64_t seed;                /* Philox key */
     uint32_t net_id;              /* stream id of this network inside its request */
     int32_t precision;            /* scrna_precision */

@@ -723,7 +723,7 @@ int orc_tauleap_simulate(const orc_network* net, const orc_kinetics* kin, const 
     const double dt = run->dt;
     const uint32_t c3 = (run->net_id << 4) | ORC_P_TAULEAP;
     const int64_t n_snap = run->snapshot_every > 0 ? run->n_steps / run->snapshot_every : 0;
-    /* decay probabilities with dilution (tauleap_binomial.jl:145, 158) and their logs */
+    /* decay probabilities with dilution (tauleap_binomial.jl:144, 157) and their logs */
     const double pdm = 1.0 - exp(-(kin->mu_m + kin->dilution) * dt);
     const double pdp = 1.0 - exp(-(kin->mu_p + kin->dilution) * dt);
     const double lqm[2] = {log(1.0 - pdm), log(pdm)};
@@ -731,7 +731,7 @@ int orc_tauleap_simulate(const orc_network* net, const orc_kinetics* kin, const 
     orc_tele tele;
     orc_tele_init(net, dt, &tele);
 
-    /* init: round(Int, steady state), half to even (tauleap_binomial.jl:59-60) */
+    /* init: round(Int, steady state), half to even (tauleap_binomial.jl:61-62) */
     int64_t* m0 = (int64_t*)malloc(sizeof(int64_t) * (size_t)G);
     int64_t* p0 = (int64_t*)malloc(sizeof(int64_t) * (size_t)G);
     for (int g = 0; g < G; ++g) {
@@ -755,22 +755,22 @@ int orc_tauleap_simulate(const orc_network* net, const orc_kinetics* kin, const 
         orc_promoter_init(&tele, G, run->seed, run->net_id << 4, (uint32_t)c, on);   /* :64-76 */
         int64_t snap = 0;
         for (int64_t s = 0; s < run->n_steps; ++s) {
-            for (int g = 0; g < G; ++g) pf[g] = (double)P[g];                     /* :80 */
-            if (tele.telegraph_any)                                               /* :83-123 */
+            for (int g = 0; g < G; ++g) pf[g] = (double)P[g];                     /* :81 */
+            if (tele.telegraph_any)                                               /* :84-123 */
                 orc_promoter_switch(&tele, G, run->seed, run->net_id << 4, (uint32_t)c, (uint32_t)s, on);
-            orc_regulation_loop(net, kin, pf, reg);                              /* :126 */
+            orc_regulation_loop(net, kin, pf, reg);                              /* :128 */
             for (int g = 0; g < G; ++g) {
                 orc_ustream st = {run->seed, (uint32_t)c, (uint32_t)s, c3, (uint32_t)g << 16, 2, {0, 0, 0, 0}};
-                double prop = reg[g] + net->basals[g];                           /* :127 */
+                double prop = reg[g] + net->basals[g];                           /* :129 */
                 if (!(prop > 0.0)) prop = 0.0;
                 if (tele.telegraph_any && !on[g]) prop = 0.0;                    /* :134-137 */
                 int64_t n_tx = orc_poisson_stream(prop * dt, &st);                /* :140 */
                 int64_t n_md = M[g] > 0 ? orc_binomial_stream_lq(M[g], pdm, lqm, &st) : 0;   /* :143-149 */
-                M[g] += n_tx - n_md;                                             /* :151 */
+                M[g] += n_tx - n_md;                                             /* :150 */
                 int64_t mpos = M[g] > 0 ? M[g] : 0;
-                int64_t n_tl = orc_poisson_stream(kin->k_t * (double)mpos * dt, &st);     /* :154 */
-                int64_t n_pd = P[g] > 0 ? orc_binomial_stream_lq(P[g], pdp, lqp, &st) : 0;   /* :157-162 */
-                P[g] += n_tl - n_pd;                                             /* :164 */
+                int64_t n_tl = orc_poisson_stream(kin->k_t * (double)mpos * dt, &st);     /* :153 */
+                int64_t n_pd = P[g] > 0 ? orc_binomial_stream_lq(P[g], pdp, lqp, &st) : 0;   /* :156-161 */
+                P[g] += n_tl - n_pd;                                             /* :163 */
             }
             if (n_snap > 0 && ((s + 1) % run->snapshot_every) == 0 && snap < n_snap) {
                 for (int g = 0; g < G; ++g) {

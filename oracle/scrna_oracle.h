@@ -116,7 +116,7 @@ int orc_cle_population(const orc_network* net, const orc_kinetics* kin, const or
 
 /* Binomial tau-leap of cells [cell_lo, cell_hi) (tauleap_binomial.jl:31-180):
  * same layouts as orc_cle_simulate; counts are returned as doubles.  init_mode
- * 0 rounds the steady state half to even (tauleap_binomial.jl:59-60).
+ * 0 rounds the steady state half to even (tauleap_binomial.jl:61-62).
  * injected must be NULL. */
 int orc_tauleap_simulate(const orc_network* net, const orc_kinetics* kin, const orc_run* run,
                          double* m_out, double* p_out, double* snaps_m, double* snaps_p,

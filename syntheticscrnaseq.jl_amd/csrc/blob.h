@@ -74,10 +74,10 @@ enum TlWord : int {
     TL_HILLN,         // n
     TL_KT,            // k_t
     TL_DT,            // dt
-    TL_PDM,           // 1 - exp(-(mu_m + dilution) dt), tauleap_binomial.jl:145
+    TL_PDM,           // 1 - exp(-(mu_m + dilution) dt), tauleap_binomial.jl:144
     TL_LQM,           // log(1 - TL_PDM)
     TL_LPM,           // log(TL_PDM)
-    TL_PDP,           // 1 - exp(-(mu_p + dilution) dt), tauleap_binomial.jl:158
+    TL_PDP,           // 1 - exp(-(mu_p + dilution) dt), tauleap_binomial.jl:157
     TL_LQP,           // log(1 - TL_PDP)
     TL_LPP,           // log(TL_PDP)
     TL_HEAD = 16

@@ -445,7 +445,7 @@ int compile_network(const scrna_network* net, const scrna_kinetics* kin, const s
     out->algorithm = run->algorithm;
     if (run->algorithm == SCRNA_ALG_BINOMIAL_TAU_LEAP) {
         // FP64 section of the tau-leap: the values the CPU restatement uses,
-        // computed with the same expressions (tauleap_binomial.jl:59-60, 145, 158).
+        // computed with the same expressions (tauleap_binomial.jl:61-62, 145, 158).
         std::vector<double> d(TL_HEAD, 0.0);
         const double pdm = 1.0 - std::exp(-(kin->mu_m + kin->dilution) * run->dt);
         const double pdp = 1.0 - std::exp(-(kin->mu_p + kin->dilution) * run->dt);

@@ -1,16 +1,16 @@
 // gfx950 binomial tau-leap stepper (src/algorithms/tauleap_binomial.jl:31-180,
-// regulation mode :transcription; GPU form ext/SyntheticscRNAseqCUDAExt.jl:48-117
-// and 132-213, whose Poisson inverse-CDF truncation and Poisson-for-binomial
-// approximation are replaced by exact draws).
+// regulation mode :transcription; GPU form ext/SyntheticscRNAseqCUDAExt.jl:230-311
+// and 317-399, whose Poisson inverse-CDF truncation and Poisson-for-binomial
+// approximation, ext:21-39, are replaced by exact draws).
 //
 // Work mapping as in the FP32 CLE stepper: a workgroup owns up to 64 cells of
 // one network (lane = cell), wave w owns genes [w*CH, w*CH+CH), and each lane
-// keeps its genes' integer mRNA and protein counts in VGPRs for the whole
-// horizon.  Per step:
+// keeps its genes' integer mRNA and protein counts on chip (in LDS) for the
+// whole horizon.  Per step:
 //   1. telegraph switching (tauleap_binomial.jl:110-121), the same Philox
 //      switch stream as the CLE;
 //   2. the Hill values of every source gene, in FP64, from the protein counts
-//      at the start of the step (:80), into LDS rows [2*slot][64] (act) and
+//      at the start of the step (:81), into LDS rows [2*slot][64] (act) and
 //      [2*slot+1][64] (rep); one workgroup barrier;
 //   3. per gene: regulation in FP64 in the reference's loop order
 //      (network.jl:158-193: additive edges in source order, then AND, then
@@ -18,7 +18,7 @@
 //      (cell, step, gene): transcription Poisson(max(reg+beta,0) dt), mRNA
 //      decay Binomial(m, 1-exp(-mu_m dt)), translation Poisson(k_t m' dt)
 //      with the updated m', protein decay Binomial(p, 1-exp(-mu_p dt))
-//      (:127-164).
+//      (:128-163).
 // All FP64 arithmetic mirrors the CPU restatement operation for operation
 // (FP contraction off), so for n = 2, where every Hill value is an exact
 // integer square, the counts agree bit for bit.
@@ -37,7 +37,7 @@ __device__ __forceinline__ uint64_t tl_u64(const uint32_t* B, int i) {
     return (uint64_t)B[i] | ((uint64_t)B[i + 1] << 32);
 }
 
-// The four draws of one gene in one step (tauleap_binomial.jl:140-164), from
+// The four draws of one gene in one step (tauleap_binomial.jl:140-163), from
 // the gene's own uniform stream.
 __device__ __forceinline__ void tl_gene_step(int& m, int& p, double lam_tx, double k_t, double dt, double pdm,
                                              const double* lqm, double pdp, const double* lqp, uint32_t cell,
@@ -46,11 +46,11 @@ __device__ __forceinline__ void tl_gene_step(int& m, int& p, double lam_tx, doub
     UStream st(cell, step, g << 16, c3t, k0, k1);
     const int n_tx = (int)poisson_sample(lam_tx, st);                                // :140
     const int n_md = m > 0 ? (int)binomial_sample(m, pdm, st, lqm) : 0;             // :143-149
-    m += n_tx - n_md;                                                                // :151
+    m += n_tx - n_md;                                                                // :150
     const int mpos = m > 0 ? m : 0;
-    const int n_tl = (int)poisson_sample(k_t * (double)mpos * dt, st);              // :154
-    const int n_pd = p > 0 ? (int)binomial_sample(p, pdp, st, lqp) : 0;             // :157-162
-    p += n_tl - n_pd;                                                                // :164
+    const int n_tl = (int)poisson_sample(k_t * (double)mpos * dt, st);              // :153
+    const int n_pd = p > 0 ? (int)binomial_sample(p, pdp, st, lqp) : 0;             // :156-161
+    p += n_tl - n_pd;                                                                // :163
 }
 
 // Dynamic LDS: Hill rows [nbuf][2S][64] doubles, then the lanes' counts
@@ -131,7 +131,7 @@ tauleap_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restric
             P(k) = (int)state_pool[off_state + (size_t)(G + g) * C + cell];
             on |= (uint32_t)(prom_pool[off_prom + (size_t)g * C + cell] != 0) << k;
         } else {
-            M(k) = (int)init_d[2 * g];                 // round(Int, steady state), tauleap_binomial.jl:59-60
+            M(k) = (int)init_d[2 * g];                 // round(Int, steady state), tauleap_binomial.jl:61-62
             P(k) = (int)init_d[2 * g + 1];
             bool o = true;                             // constitutive genes are always ON
             if (tele && (tf & 2u)) {                   // tauleap_binomial.jl:64-76
@@ -213,7 +213,7 @@ tauleap_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restric
                 }
                 q += 2 + ns;
             }
-            double prop = reg + beta_d[g];                                       // :127
+            double prop = reg + beta_d[g];                                       // :129
             if (!(prop > 0.0)) prop = 0.0;
             if (tele && !((on >> k) & 1u)) prop = 0.0;                           // :134-137
             int mk = M(k), pk = P(k);

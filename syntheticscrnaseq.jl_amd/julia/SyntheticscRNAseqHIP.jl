@@ -15,7 +15,7 @@ Set `ENV["SCRNA_LIB"]` to the library path if it is not next to this file.
 module SyntheticscRNAseqHIP
 
 using SyntheticscRNAseq
-using SyntheticscRNAseq: GeneNetwork, KineticParams, PopulationConfig, CaptureModel, CLE
+using SyntheticscRNAseq: GeneNetwork, KineticParams, PopulationConfig, CaptureModel, CLE, BinomialTauLeap
 using Random
 
 const LIB = get(ENV, "SCRNA_LIB", joinpath(@__DIR__, "..", "libscrna.so"))
@@ -55,6 +55,7 @@ end
 struct CRun
     dt::Float64; T::Float64; cell_num::Int64; readout::Int32; init_mode::Int32; seed::UInt64
     net_id::UInt32; precision::Int32; snapshot_every::Int64; injected_noise::Ptr{Float32}
+    algorithm::Int32; reserved0::Int32
 end
 
 struct COutput
@@ -119,9 +120,13 @@ end
 _ccap(c::Union{Nothing,CaptureModel}) = c === nothing ? CCapture(0, 0, 0, 0) :
     CCapture(1, c.efficiency, c.efficiency_std, READOUT[c.readout])
 
-function _crun(alg::CLE, cell_num, T, readout, seed, net_id; init=:steady, precision=:fp32, snapshot_every=0)
+const GpuAlg = Union{CLE,BinomialTauLeap}
+_algorithm(::CLE) = Int32(0)
+_algorithm(::BinomialTauLeap) = Int32(1)
+
+function _crun(alg::GpuAlg, cell_num, T, readout, seed, net_id; init=:steady, precision=:fp32, snapshot_every=0)
     CRun(alg.dt, T, Int64(cell_num), READOUT[readout], init === :zeros ? 1 : 0, UInt64(seed), UInt32(net_id),
-         PRECISION[precision], Int64(snapshot_every), Ptr{Float32}(0))
+         PRECISION[precision], Int64(snapshot_every), Ptr{Float32}(0), _algorithm(alg), Int32(0))
 end
 
 # ---- drop-in methods -------------------------------------------------------------
@@ -134,6 +139,13 @@ function SyntheticscRNAseq.simulate(net::GeneNetwork, alg::CLE, kin::KineticPara
     _simulate(net, alg, kin, nothing, nothing; cell_num, T, readout, seed, init, precision)[1]
 end
 
+# ext/SyntheticscRNAseqCUDAExt.jl:230-311 (exact Poisson / binomial draws; counts as Float64)
+function SyntheticscRNAseq.simulate(net::GeneNetwork, alg::BinomialTauLeap, kin::KineticParams, ::Val{:gpu};
+                                    cell_num::Int=1000, T::Float64=50.0, readout::Symbol=:protein,
+                                    seed::UInt64=rand(Random.RandomDevice(), UInt64), init::Symbol=:steady)
+    _simulate(net, alg, kin, nothing, nothing; cell_num, T, readout, seed, init)[1]
+end
+
 # ext/SyntheticscRNAseqCUDAExt.jl:409-529 (division on the device, no host round trip)
 function SyntheticscRNAseq.simulate(net::GeneNetwork, alg::CLE, kin::KineticParams, ::Val{:gpu},
                                     pop::PopulationConfig; T::Float64=50.0, readout::Symbol=:protein,
@@ -142,7 +154,7 @@ function SyntheticscRNAseq.simulate(net::GeneNetwork, alg::CLE, kin::KineticPara
 end
 
 # North-star keyword form: simulate(net, kin; algorithm=CLE(dt), capture=CaptureModel(...)) -> counts
-function SyntheticscRNAseq.simulate(net::GeneNetwork, kin::KineticParams; algorithm::CLE,
+function SyntheticscRNAseq.simulate(net::GeneNetwork, kin::KineticParams; algorithm::GpuAlg,
                                     capture::CaptureModel, cell_num::Int=1000, T::Float64=50.0,
                                     seed::UInt64=rand(Random.RandomDevice(), UInt64), init::Symbol=:steady,
                                     precision::Symbol=:fp32, population=nothing)
@@ -174,8 +186,8 @@ function _simulate(net, alg, kin, pop, cap; cell_num, T, readout, seed, init=:st
     return Y, Matrix{Int}(counts)
 end
 
-# ext/SyntheticscRNAseqCUDAExt.jl:132-213 (networks may differ in G here)
-function SyntheticscRNAseq.simulate_gpu_batch(networks::Vector{GeneNetwork}, alg::CLE, kin::KineticParams;
+# ext/SyntheticscRNAseqCUDAExt.jl:132-213 and 317-399 (networks may differ in G here)
+function SyntheticscRNAseq.simulate_gpu_batch(networks::Vector{GeneNetwork}, alg::GpuAlg, kin::KineticParams;
                                               cell_num::Int=1000, T::Float64=50.0, readout::Symbol=:protein,
                                               seed::UInt64=rand(Random.RandomDevice(), UInt64))
     n = length(networks)

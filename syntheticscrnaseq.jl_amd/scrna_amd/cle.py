@@ -69,7 +69,7 @@ def _run(alg, cell_num: int, T: float, readout: str, init: str, seed, net_id: in
     if readout not in _lib.READOUT:
         raise ValueError("readout must be 'mrna', 'protein' or 'both'")
     if init not in ("steady", "zeros"):
-        raise ValueError("init must be 'steady' (the steady state, cle.jl:52-53 / tauleap_binomial.jl:59-60) "
+        raise ValueError("init must be 'steady' (the steady state, cle.jl:52-53 / tauleap_binomial.jl:61-62) "
                          "or 'zeros'")
     if precision not in _lib.PRECISION:
         raise ValueError("precision must be 'fp32', 'bf16' or 'bf16x3'")

@@ -8,9 +8,9 @@ the reference, which cannot run here (no Julia).
   pins transcription, decay and their order.  The protein recursion
   p' = Binomial(p, b) + Poisson(k_t dt m') is linear, so its stationary mean,
   variance and covariance with m have closed forms at the actual dt; the
-  covariance separates "translation reads the updated m" (tauleap_binomial.jl:154)
+  covariance separates "translation reads the updated m" (tauleap_binomial.jl:153)
   from reading the old one by a factor a.
-* Initial counts round the steady state half to even (tauleap_binomial.jl:59-60).
+* Initial counts round the steady state half to even (tauleap_binomial.jl:61-62).
 """
 import math
 
@@ -94,7 +94,7 @@ def test_immigration_death_is_exactly_poisson_and_protein_moments():
 
 
 def test_dilution_enters_both_decays():
-    """tauleap_binomial.jl:145 and :158 use mu + dilution."""
+    """tauleap_binomial.jl:144 and :158 use mu + dilution."""
     beta, mu_m, mu_p, k_t, dt, dil = 10.0, 0.5, 0.3, 1.0, 0.1, 0.2
     net = sc.GeneNetwork(1, np.array([beta]), np.zeros((1, 1)))
     kin = sc.KineticParams(k_t=k_t, K_d=10.0, n=2.0, mu_m=mu_m, mu_p=mu_p, dilution=dil)
