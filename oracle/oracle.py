@@ -72,6 +72,8 @@ def load():
         lib.orc_cle_population.restype = C.c_int
         lib.orc_tauleap_simulate.argtypes = lib.orc_cle_simulate.argtypes
         lib.orc_tauleap_simulate.restype = C.c_int
+        lib.orc_tauleap_population.argtypes = lib.orc_cle_population.argtypes
+        lib.orc_tauleap_population.restype = C.c_int
         lib.orc_poisson.argtypes = [C.c_double, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64]
         lib.orc_poisson.restype = C.c_int64
         lib.orc_binomial.argtypes = [C.c_int64, C.c_double, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64]
@@ -193,7 +195,14 @@ def cle(net, kin, *, dt, T=None, n_steps=None, cell_num, cell_lo=0, cell_hi=None
     return dict(m=m, p=p, snaps_m=sm[:n_snap], snaps_p=sp[:n_snap], promoter=pr)
 
 
-def cle_population(net, kin, pop, *, dt, T, cell_num, seed=0, net_id=0, init="steady", injected=None):
+def tauleap_population(net, kin, pop, *, dt, T, cell_num, seed=0, net_id=0, init="steady"):
+    """FP64 binomial tau-leap with Moran population (tauleap_binomial.jl:195-261)."""
+    return cle_population(net, kin, pop, dt=dt, T=T, cell_num=cell_num, seed=seed, net_id=net_id, init=init,
+                          _fn="orc_tauleap_population")
+
+
+def cle_population(net, kin, pop, *, dt, T, cell_num, seed=0, net_id=0, init="steady", injected=None,
+                   _fn="orc_cle_population"):
     lib = load()
     n_steps = int(np.ceil(T / dt))
     s, keep = _net(net)
@@ -213,7 +222,7 @@ def cle_population(net, kin, pop, *, dt, T, cell_num, seed=0, net_id=0, init="st
     p = np.empty((G, cell_num))
     V = np.empty(cell_num)
     pr = np.empty((G, cell_num), dtype=np.uint8)
-    rc = lib.orc_cle_population(C.byref(s), C.byref(_kin(kin)), C.byref(run), C.byref(op),
+    rc = getattr(lib, _fn)(C.byref(s), C.byref(_kin(kin)), C.byref(run), C.byref(op),
                                 m.ctypes.data_as(_dp), p.ctypes.data_as(_dp), V.ctypes.data_as(_dp),
                                 pr.ctypes.data_as(_up))
     if rc:

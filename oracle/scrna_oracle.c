@@ -551,6 +551,64 @@ int orc_capture(const double* X, int64_t cells, int32_t G, int64_t ld, double ef
     return 0;
 }
 
+/* One division check (population.jl:157-205; cle.jl:240-272): every cell
+ * with V > V_div divides, then each daughter replaces a cell (Moran).
+ * Daughter buffers are caller-provided ([N] and [N*G]). */
+static void orc_division_check(const orc_population* pop, int G, int64_t N, uint64_t seed, uint32_t c3base,
+                               int64_t s, double* M, double* P, double* V, uint8_t* on, int64_t* div_idx,
+                               double* dM, double* dP, double* dV, uint8_t* dOn) {
+    int64_t nd = 0;
+    for (int64_t c = 0; c < N; ++c) if (V[c] > pop->V_div) div_idx[nd++] = c;   /* cle.jl:241 */
+    /* Phase 1: partition mothers, buffer daughters (population.jl:166-192) */
+    for (int64_t k = 0; k < nd; ++k) {
+        int64_t c = div_idx[k];
+        for (int g = 0; g < G; ++g) {
+            for (int sp = 0; sp < 2; ++sp) {
+                double* X = sp == 0 ? &M[c * G + g] : &P[c * G + g];
+                double* D = sp == 0 ? &dM[k * G + g] : &dP[k * G + g];
+                if (pop->partition == 0) {                      /* cle.jl:251-257 */
+                    double h = *X / 2.0; *X = h; *D = h;
+                } else {
+                    int64_t n = orc_round_even(*X);
+                    if (n < 0) n = 0;
+                    int64_t d = 0;
+                    if (n > 0) {
+                        uint32_t c3 = c3base | ORC_P_PARTITION;
+                        /* sub-stream per (species, gene) in counter word 2 high bits */
+                        orc_ustream st = {seed, (uint32_t)c, (uint32_t)s, c3,
+                                          ((uint32_t)(g * 2 + sp)) << 16, 2, {0, 0, 0, 0}};
+                        d = orc_binomial_stream(n, 0.5, &st);
+                    }
+                    *X = (double)(n - d); *D = (double)d;       /* population.jl:174-176 */
+                }
+            }
+            dOn[k * G + g] = on[c * G + g];                     /* daughter inherits promoter state */
+        }
+        V[c] /= 2.0;                                            /* population.jl:190 */
+        dV[k] = V[c];
+    }
+    /* Phase 2: Moran replacement in divider order (population.jl:194-202) */
+    for (int64_t k = 0; k < nd; ++k) {
+        int64_t mother = div_idx[k];
+        uint32_t u[4];
+        orc_draw(seed, (uint32_t)mother, (uint32_t)s, 0, c3base | ORC_P_REPLACE, u);
+        uint64_t x = ((uint64_t)u[0] << 32) | u[1];
+        int64_t replaced;
+        if (pop->replace_mode == 1 && N > 1) {                  /* cle.jl:265-268 */
+            replaced = (int64_t)(((unsigned __int128)x * (uint64_t)(N - 1)) >> 64);
+            if (replaced >= mother) replaced += 1;
+        } else {
+            replaced = (int64_t)(((unsigned __int128)x * (uint64_t)N) >> 64);
+        }
+        for (int g = 0; g < G; ++g) {
+            M[replaced * G + g] = dM[k * G + g];
+            P[replaced * G + g] = dP[k * G + g];
+            on[replaced * G + g] = dOn[k * G + g];
+        }
+        V[replaced] = dV[k];
+    }
+}
+
 /* ---------------------------------------------------------------------- */
 /* Population CLE, cle.jl:143-284 + population.jl:157-205                  */
 /* ---------------------------------------------------------------------- */
@@ -627,58 +685,8 @@ int orc_cle_population(const orc_network* net, const orc_kinetics* kin, const or
             }
             V[c] *= growth;                                             /* cle.jl:237 */
         }
-        if (((s + 1) % pop->div_check_interval) == 0) {                 /* cle.jl:240 */
-            int64_t nd = 0;
-            for (int64_t c = 0; c < N; ++c) if (V[c] > pop->V_div) div_idx[nd++] = c;   /* cle.jl:241 */
-            /* Phase 1: partition mothers, buffer daughters (population.jl:423-444) */
-            for (int64_t k = 0; k < nd; ++k) {
-                int64_t c = div_idx[k];
-                for (int g = 0; g < G; ++g) {
-                    for (int sp = 0; sp < 2; ++sp) {
-                        double* X = sp == 0 ? &M[c * G + g] : &P[c * G + g];
-                        double* D = sp == 0 ? &dM[k * G + g] : &dP[k * G + g];
-                        if (pop->partition == 0) {                      /* cle.jl:251-257 */
-                            double h = *X / 2.0; *X = h; *D = h;
-                        } else {
-                            int64_t n = orc_round_even(*X);
-                            if (n < 0) n = 0;
-                            int64_t d = 0;
-                            if (n > 0) {
-                                uint32_t c3 = c3base | ORC_P_PARTITION;
-                                /* sub-stream per (species, gene) in counter word 2 high bits */
-                                orc_ustream st = {run->seed, (uint32_t)c, (uint32_t)s, c3,
-                                                  ((uint32_t)(g * 2 + sp)) << 16, 2, {0, 0, 0, 0}};
-                                d = orc_binomial_stream(n, 0.5, &st);
-                            }
-                            *X = (double)(n - d); *D = (double)d;       /* population.jl:426-428 */
-                        }
-                    }
-                    dOn[k * G + g] = on[c * G + g];                     /* daughter inherits promoter state */
-                }
-                V[c] /= 2.0;                                            /* population.jl:442 */
-                dV[k] = V[c];
-            }
-            /* Phase 2: Moran replacement in divider order (population.jl:447-454) */
-            for (int64_t k = 0; k < nd; ++k) {
-                int64_t mother = div_idx[k];
-                uint32_t u[4];
-                orc_draw(run->seed, (uint32_t)mother, (uint32_t)s, 0, c3base | ORC_P_REPLACE, u);
-                uint64_t x = ((uint64_t)u[0] << 32) | u[1];
-                int64_t replaced;
-                if (pop->replace_mode == 1 && N > 1) {                  /* cle.jl:265-268 */
-                    replaced = (int64_t)(((unsigned __int128)x * (uint64_t)(N - 1)) >> 64);
-                    if (replaced >= mother) replaced += 1;
-                } else {
-                    replaced = (int64_t)(((unsigned __int128)x * (uint64_t)N) >> 64);
-                }
-                for (int g = 0; g < G; ++g) {
-                    M[replaced * G + g] = dM[k * G + g];
-                    P[replaced * G + g] = dP[k * G + g];
-                    on[replaced * G + g] = dOn[k * G + g];
-                }
-                V[replaced] = dV[k];
-            }
-        }
+        if (((s + 1) % pop->div_check_interval) == 0)                   /* cle.jl:240 */
+            orc_division_check(pop, G, N, run->seed, c3base, s, M, P, V, on, div_idx, dM, dP, dV, dOn);
     }
     for (int64_t c = 0; c < N; ++c) {
         for (int g = 0; g < G; ++g) {
@@ -789,6 +797,97 @@ int orc_tauleap_simulate(const orc_network* net, const orc_kinetics* kin, const 
         free(M); free(pf); free(on);
     }
     free(m0); free(p0);
+    orc_tele_free(&tele);
+    return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Binomial tau-leap with population dynamics                              */
+/* (tauleap_binomial.jl:195-261; population.jl:96-116, 128-139, 157-205)   */
+/* ---------------------------------------------------------------------- */
+int orc_tauleap_population(const orc_network* net, const orc_kinetics* kin, const orc_run* run,
+                           const orc_population* pop, double* m_out, double* p_out, double* V_out,
+                           uint8_t* promoter_out) {
+    if (orc_check(net, kin, run) || !pop || pop->div_check_interval <= 0 || run->injected) return 1;
+    if (pop->partition != 1) return 1;          /* counts stay integers only with binomial partitioning */
+    const int G = net->G;
+    const int64_t N = run->cell_num;
+    if (N <= 0) return 1;
+    const double dt = run->dt;
+    const uint32_t c3base = run->net_id << 4, c3t = c3base | ORC_P_TAULEAP;
+    /* intrinsic decay only (tauleap_binomial.jl:228, 242); growth factor (population.jl:136) */
+    const double pdm = 1.0 - exp(-kin->mu_m * dt);
+    const double pdp = 1.0 - exp(-kin->mu_p * dt);
+    const double lqm[2] = {log(1.0 - pdm), log(pdm)};
+    const double lqp[2] = {log(1.0 - pdp), log(pdp)};
+    const double growth = exp(pop->growth_rate * dt);
+    orc_tele tele;
+    orc_tele_init(net, dt, &tele);
+
+    double* M = (double*)malloc(sizeof(double) * (size_t)G * (size_t)N);
+    double* P = (double*)malloc(sizeof(double) * (size_t)G * (size_t)N);
+    double* V = (double*)malloc(sizeof(double) * (size_t)N);
+    uint8_t* on = (uint8_t*)malloc((size_t)G * (size_t)N);
+    double* reg = (double*)malloc(sizeof(double) * (size_t)G);
+    double* pf = (double*)malloc(sizeof(double) * (size_t)G);
+    int64_t* div_idx = (int64_t*)malloc(sizeof(int64_t) * (size_t)N);
+    double* dM = (double*)malloc(sizeof(double) * (size_t)G * (size_t)N);
+    double* dP = (double*)malloc(sizeof(double) * (size_t)G * (size_t)N);
+    double* dV = (double*)malloc(sizeof(double) * (size_t)N);
+    uint8_t* dOn = (uint8_t*)malloc((size_t)G * (size_t)N);
+
+    for (int64_t c = 0; c < N; ++c) {
+        for (int g = 0; g < G; ++g) {
+            double mss = net->basals[g] / (kin->mu_m + kin->dilution);         /* kinetics.jl:65-70 */
+            double pss = kin->k_t * mss / (kin->mu_p + kin->dilution);
+            M[c * G + g] = run->init_mode == 0 ? (double)orc_round_even(mss) : 0.0;   /* population.jl:104-105 */
+            P[c * G + g] = run->init_mode == 0 ? (double)orc_round_even(pss) : 0.0;
+        }
+        uint32_t u[4];
+        orc_draw(run->seed, (uint32_t)c, 0, 0, c3base | ORC_P_VINIT, u);
+        V[c] = pop->V_init_lo + (pop->V_init_hi - pop->V_init_lo) * orc_u52(u[0], u[1]);   /* population.jl:108-109 */
+        orc_promoter_init(&tele, G, run->seed, c3base, (uint32_t)c, on + c * G);
+    }
+
+    for (int64_t s = 0; s < run->n_steps; ++s) {
+        for (int64_t c = 0; c < N; ++c) {
+            double* Mc = M + c * G; double* Pc = P + c * G; uint8_t* oc = on + c * G;
+            if (tele.telegraph_any)
+                orc_promoter_switch(&tele, G, run->seed, c3base, (uint32_t)c, (uint32_t)s, oc);
+            for (int g = 0; g < G; ++g) pf[g] = Pc[g];                          /* counts, :213-215 */
+            orc_regulation_loop(net, kin, pf, reg);                             /* :218 */
+            for (int g = 0; g < G; ++g) {
+                orc_ustream st = {run->seed, (uint32_t)c, (uint32_t)s, c3t, (uint32_t)g << 16, 2, {0, 0, 0, 0}};
+                double prop = reg[g] + net->basals[g];
+                if (!(prop > 0.0)) prop = 0.0;
+                prop = prop * V[c];                                             /* :221 */
+                if (tele.telegraph_any && !oc[g]) prop = 0.0;
+                int64_t m = (int64_t)Mc[g], p = (int64_t)Pc[g];
+                int64_t n_tx = orc_poisson_stream(prop * dt, &st);               /* :224 */
+                int64_t n_md = m > 0 ? orc_binomial_stream_lq(m, pdm, lqm, &st) : 0;   /* :227-233 */
+                m += n_tx - n_md;
+                int64_t mpos = m > 0 ? m : 0;
+                int64_t n_tl = orc_poisson_stream(kin->k_t * (double)mpos * dt, &st);   /* :237-238 */
+                int64_t n_pd = p > 0 ? orc_binomial_stream_lq(p, pdp, lqp, &st) : 0;   /* :241-247 */
+                p += n_tl - n_pd;
+                Mc[g] = (double)m;
+                Pc[g] = (double)p;
+            }
+        }
+        for (int64_t c = 0; c < N; ++c) V[c] *= growth;                         /* :253; population.jl:136-137 */
+        if (((s + 1) % pop->div_check_interval) == 0)                            /* :256-258 */
+            orc_division_check(pop, G, N, run->seed, c3base, s, M, P, V, on, div_idx, dM, dP, dV, dOn);
+    }
+    for (int64_t c = 0; c < N; ++c) {
+        for (int g = 0; g < G; ++g) {
+            if (m_out) m_out[(size_t)g * N + c] = M[c * G + g];
+            if (p_out) p_out[(size_t)g * N + c] = P[c * G + g];
+            if (promoter_out) promoter_out[(size_t)g * N + c] = on[c * G + g];
+        }
+        if (V_out) V_out[c] = V[c];
+    }
+    free(M); free(P); free(V); free(on); free(reg); free(pf); free(div_idx);
+    free(dM); free(dP); free(dV); free(dOn);
     orc_tele_free(&tele);
     return 0;
 }

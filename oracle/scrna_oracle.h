@@ -122,6 +122,14 @@ int orc_tauleap_simulate(const orc_network* net, const orc_kinetics* kin, const 
                          double* m_out, double* p_out, double* snaps_m, double* snaps_p,
                          uint8_t* promoter_out);
 
+/* Binomial tau-leap with Moran population dynamics over all cell_num cells
+ * (tauleap_binomial.jl:195-261): Hill on counts, transcription scaled by the
+ * cell volume, intrinsic decay only, binomial partitioning (pop->partition
+ * must be 1) and Moran replacement.  V_out: [cell_num]. */
+int orc_tauleap_population(const orc_network* net, const orc_kinetics* kin, const orc_run* run,
+                           const orc_population* pop, double* m_out, double* p_out,
+                           double* V_out, uint8_t* promoter_out);
+
 /* Poisson(lam) drawn from the Philox stream with the given counter base
  * (blocks counted in ctr[2]).  Exposed for tests. */
 int64_t orc_poisson(double lam, uint32_t ctr0, uint32_t ctr1, uint32_t c3, uint64_t seed);

@@ -52,8 +52,8 @@ hipError_t set_tauleap_lds_limit(size_t bytes);
 int tauleap_blocks_per_cu(int NW, size_t lds_bytes);
 int tauleap_max_waves();
 hipError_t launch_tauleap(int CH, int NW, int n_tiles, const uint32_t* blobs, const Tile* tiles, float* state_pool,
-                          float* snap_pool, uint8_t* prom_pool, int s_begin, int s_end, int nbuf, size_t lds_bytes,
-                          int* sched, int n_seg, long long wait_ticks, hipStream_t stream);
+                          float* snap_pool, uint8_t* prom_pool, double* vol_pool, int s_begin, int s_end, int nbuf,
+                          size_t lds_bytes, int* sched, int n_seg, long long wait_ticks, hipStream_t stream);
 hipError_t launch_capture_f32(int n_jobs, int64_t max_elems, const CaptureJob* jobs, const float* src,
                               int32_t* dst, hipStream_t stream);
 hipError_t launch_capture_f64(int n_jobs, int64_t max_elems, const CaptureJob* jobs, const double* src,

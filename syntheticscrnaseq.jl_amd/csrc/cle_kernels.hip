@@ -396,7 +396,7 @@ pop_init_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ til
     vol_pool[(size_t)u64_at(B, H_OUT_V_LO) + cell] = lo + (hi - lo) * u52(u.x, u.y);
 }
 
-// Division phase 1 (population.jl:414-444 / cle.jl:241-261): every cell with
+// Division phase 1 (population.jl:157-192 / cle.jl:241-261): every cell with
 // V > V_div after step s_div splits its molecules with its daughter -- binomially
 // (round half-even, then Binomial(n, 1/2)) or by exact halving -- halves its
 // volume, buffers the daughter, and claims a uniformly drawn replacement slot.
@@ -457,7 +457,7 @@ pop_divide_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ t
     if ((mode & 2u) && C > 1) {                        // cle.jl:265-268
         r = (int64_t)__umul64hi(x, (uint64_t)(C - 1));
         if (r >= cell) r += 1;
-    } else {                                           // population.jl:448
+    } else {                                           // population.jl:196
         r = (int64_t)__umul64hi(x, (uint64_t)C);
     }
     atomicMax(&claim[off_vol + r], cell + 1);

@@ -174,8 +174,9 @@ int validate_inputs(const scrna_network* net, const scrna_kinetics* kin, const s
             return set_error(SCRNA_ERR_UNSUPPORTED, "the tau-leap computes propensities in FP64; precision must be 0");
         if (run->injected_noise)
             return set_error(SCRNA_ERR_INVALID, "injected noise applies to the CLE only");
-        if (pop && pop->enabled)
-            return set_error(SCRNA_ERR_UNSUPPORTED, "population dynamics are implemented for the CLE only");
+        if (pop && pop->enabled && pop->partition != 1)
+            return set_error(SCRNA_ERR_UNSUPPORTED,
+                             "the tau-leap keeps integer counts: population runs need binomial partitioning");
     }
     if (run->precision == SCRNA_PREC_FP32_UNFUSED) {
         bool tele = false;
@@ -447,8 +448,12 @@ int compile_network(const scrna_network* net, const scrna_kinetics* kin, const s
         // FP64 section of the tau-leap: the values the CPU restatement uses,
         // computed with the same expressions (tauleap_binomial.jl:61-62, 145, 158).
         std::vector<double> d(TL_HEAD, 0.0);
-        const double pdm = 1.0 - std::exp(-(kin->mu_m + kin->dilution) * run->dt);
-        const double pdp = 1.0 - std::exp(-(kin->mu_p + kin->dilution) * run->dt);
+        // with population dynamics decay is intrinsic only, dilution being emergent
+        // (tauleap_binomial.jl:228, 242); otherwise mu + dilution (:144, 157)
+        const double pdm = population ? 1.0 - std::exp(-kin->mu_m * run->dt)
+                                      : 1.0 - std::exp(-(kin->mu_m + kin->dilution) * run->dt);
+        const double pdp = population ? 1.0 - std::exp(-kin->mu_p * run->dt)
+                                      : 1.0 - std::exp(-(kin->mu_p + kin->dilution) * run->dt);
         d[TL_KN] = std::pow(kin->K_d, kin->n);
         d[TL_HILLN] = kin->n;
         d[TL_KT] = kin->k_t;
@@ -866,8 +871,8 @@ int launch_steps(scrna_plan* P, hipStream_t st, int s_begin, int s_end, bool seg
                                        sched, nseg, P->wait_ticks, st));
             else
                 HIP_TRY(launch_tauleap(lg.CH, lg.NW, lg.n_tiles, P->d_blobs, P->d_tiles + lg.tile0, P->d_state,
-                                       P->d_snap, P->d_prom, s_begin, s_end, lg.nbuf, lg.lds, sched, nseg,
-                                       P->wait_ticks, st));
+                                       P->d_snap, P->d_prom, P->d_vol, s_begin, s_end, lg.nbuf, lg.lds, sched,
+                                       nseg, P->wait_ticks, st));
         } else
             HIP_TRY(launch_dense(lg.mode, lg.tele, lg.inject, lg.NW, lg.n_tiles, P->d_blobs, P->d_tiles + lg.tile0,
                                  P->d_wpool, P->d_state, P->d_snap, P->d_prom, P->d_inj, s_begin, s_end, lg.lds,

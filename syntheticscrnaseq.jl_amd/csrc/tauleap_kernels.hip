@@ -63,8 +63,8 @@ constexpr int kTauLeapMaxWaves = 8;
 __global__ void __launch_bounds__(64 * kTauLeapMaxWaves)
 tauleap_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ tiles,
                        float* __restrict__ state_pool, float* __restrict__ snap_pool,
-                       uint8_t* __restrict__ prom_pool, int s_begin, int s_end, int nbuf, int CH,
-                       int* __restrict__ sched, int n_tiles, int n_seg, long long wait_ticks) {
+                       uint8_t* __restrict__ prom_pool, double* __restrict__ vol_pool, int s_begin, int s_end,
+                       int nbuf, int CH, int* __restrict__ sched, int n_tiles, int n_seg, long long wait_ticks) {
 #pragma clang fp contract(off)
     extern __shared__ double tl_lds[];
     __shared__ int sh_ticket, sh_fail;
@@ -88,6 +88,8 @@ tauleap_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restric
     const uint32_t c3t = (B[H_NETID] << 4) | kTauLeap;
     const bool hill2 = (flags & F_HILL2) != 0;
     const bool tele = (flags & F_TELE) != 0;
+    const bool pop = (flags & F_POP) != 0;
+    const double growth = __longlong_as_double((long long)tl_u64(B, H_GROWTH_LO));
     const double* __restrict__ D = (const double*)(B + B[H_OFF_TL]);
     const double Kn = D[TL_KN], hill_n = D[TL_HILLN], k_t = D[TL_KT], dt = D[TL_DT];
     const double pdm = D[TL_PDM], pdp = D[TL_PDP];
@@ -108,6 +110,7 @@ tauleap_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restric
     const size_t off_state = (size_t)tl_u64(B, H_OUT_FINAL_LO);
     const size_t off_snap = (size_t)tl_u64(B, H_OUT_SNAP_LO);
     const size_t off_prom = (size_t)tl_u64(B, H_OUT_PROM_LO);
+    const size_t off_vol = (size_t)tl_u64(B, H_OUT_V_LO);
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -144,6 +147,8 @@ tauleap_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restric
         }
     }
     if (!tele) sw_mask = 0;
+    // cell volume (population runs): written by the population init kernel or a previous phase
+    double V = pop ? vol_pool[off_vol + cell] : 1.0;
 
     for (int s = s_start; s < s_stop; ++s) {
         double* __restrict__ H = tl_lds + (nbuf == 2 ? (s & 1) * rows * 64 : 0);
@@ -215,6 +220,7 @@ tauleap_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restric
             }
             double prop = reg + beta_d[g];                                       // :129
             if (!(prop > 0.0)) prop = 0.0;
+            if (pop) prop = prop * V;                                            // :221
             if (tele && !((on >> k) & 1u)) prop = 0.0;                           // :134-137
             int mk = M(k), pk = P(k);
             tl_gene_step(mk, pk, prop * dt, k_t, dt, pdm, lqm, pdp, lqp, (uint32_t)cell, (uint32_t)s, (uint32_t)g,
@@ -222,6 +228,7 @@ tauleap_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restric
             M(k) = mk;
             P(k) = pk;
         }
+        if (pop) V *= growth;                                                    // :253, population.jl:136-137
         if (nbuf == 1) __syncthreads();
 
         if (snap_every > 0 && ((s + 1) % snap_every) == 0 && live) {
@@ -242,6 +249,7 @@ tauleap_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restric
             state_pool[off_state + (size_t)(G + g) * C + cell] = (float)P(k);
             prom_pool[off_prom + (size_t)g * C + cell] = (uint8_t)((on >> k) & 1u);
         }
+        if (pop && wave == 0) vol_pool[off_vol + cell] = V;
     }
     segment_done(sched, sl);
 }
@@ -267,15 +275,15 @@ int tauleap_blocks_per_cu(int NW, size_t lds_bytes) {
 }
 
 hipError_t launch_tauleap(int CH, int NW, int n_tiles, const uint32_t* blobs, const Tile* tiles, float* state_pool,
-                          float* snap_pool, uint8_t* prom_pool, int s_begin, int s_end, int nbuf, size_t lds_bytes,
-                          int* sched, int n_seg, long long wait_ticks, hipStream_t stream) {
+                          float* snap_pool, uint8_t* prom_pool, double* vol_pool, int s_begin, int s_end, int nbuf,
+                          size_t lds_bytes, int* sched, int n_seg, long long wait_ticks, hipStream_t stream) {
     if (n_tiles <= 0) return hipSuccess;
     if (sched == nullptr) n_seg = 1;
     if (n_seg < 1 || (long long)n_tiles * n_seg > INT32_MAX || NW < 1 || NW > kTauLeapMaxWaves || CH < 1 || CH > 32)
         return hipErrorInvalidValue;
     const dim3 grid((unsigned)(n_tiles * n_seg)), block(64 * NW);
     hipLaunchKernelGGL(tauleap_stepper_kernel, grid, block, lds_bytes, stream, blobs, tiles, state_pool, snap_pool,
-                       prom_pool, s_begin, s_end, nbuf, CH, sched, n_tiles, n_seg, wait_ticks);
+                       prom_pool, vol_pool, s_begin, s_end, nbuf, CH, sched, n_tiles, n_seg, wait_ticks);
     return hipGetLastError();
 }
 

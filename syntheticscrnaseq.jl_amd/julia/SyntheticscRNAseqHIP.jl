@@ -146,6 +146,14 @@ function SyntheticscRNAseq.simulate(net::GeneNetwork, alg::BinomialTauLeap, kin:
     _simulate(net, alg, kin, nothing, nothing; cell_num, T, readout, seed, init)[1]
 end
 
+# ext/SyntheticscRNAseqCUDAExt.jl:543-680 (counts, binomial partitioning, division on the device;
+# the Hill functions read counts as in tauleap_binomial.jl:213-221)
+function SyntheticscRNAseq.simulate(net::GeneNetwork, alg::BinomialTauLeap, kin::KineticParams, ::Val{:gpu},
+                                    pop::PopulationConfig; T::Float64=50.0, readout::Symbol=:protein,
+                                    seed::UInt64=rand(Random.RandomDevice(), UInt64))
+    _simulate(net, alg, kin, pop, nothing; cell_num=pop.cell_num, T, readout, seed, partition=:binomial)[1]
+end
+
 # ext/SyntheticscRNAseqCUDAExt.jl:409-529 (division on the device, no host round trip)
 function SyntheticscRNAseq.simulate(net::GeneNetwork, alg::CLE, kin::KineticParams, ::Val{:gpu},
                                     pop::PopulationConfig; T::Float64=50.0, readout::Symbol=:protein,

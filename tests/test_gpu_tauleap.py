@@ -150,4 +150,33 @@ def test_unsupported_combinations_raise(ctx):
     with pytest.raises(sc.UnsupportedError):
         sc.simulate(net, sc.BinomialTauLeap(0.1), KIN, cell_num=10, T=1.0, precision="bf16")
     with pytest.raises(sc.UnsupportedError):
-        sc.simulate(net, sc.BinomialTauLeap(0.1), KIN, "gpu", sc.PopulationConfig(cell_num=10), T=1.0)
+        sc.simulate(net, sc.BinomialTauLeap(0.1), KIN, "gpu", sc.PopulationConfig(cell_num=10, partition="halve"),
+                    T=1.0)
+
+
+@pytest.mark.parametrize("telegraph", [False, True])
+def test_population_matches_oracle(ctx, telegraph):
+    """Tau-leap with Moran population (tauleap_binomial.jl:195-261): volume-scaled
+    transcription, intrinsic decay, binomial partitioning, uniform replacement;
+    division on the device."""
+    rng = np.random.default_rng(17 + telegraph)
+    net = sc.samplers.mixed_network(10, rng, n_tf=(2, 3), targets_per_tf=(2, 4), n_and=(1, 1), n_or=(1, 1),
+                                    telegraph=telegraph)
+    pop = sc.PopulationConfig(cell_num=300, growth_rate=0.05, V_div=2.0, V_init=(0.8, 1.2), div_check_interval=5)
+    dt, T, seed = 0.1, 20.0, 13
+    res = sc.simulate(net, sc.BinomialTauLeap(dt), KIN, "gpu", pop, T=T, readout="both", seed=seed, full=True)
+    ref = orc.tauleap_population(net, KIN, pop, dt=dt, T=T, cell_num=300, seed=seed)
+    assert np.array_equal(res.volumes, ref["V"])
+    assert np.array_equal(res.y, both(ref))
+
+
+def test_population_reference_band(ctx):
+    """test_analytical_moments.jl:233-252: N cells kept, counts >= 0, mean protein
+    above half the no-population value."""
+    net = sc.GeneNetwork(1, np.array([2.0]), np.zeros((1, 1)))
+    kin = sc.KineticParams(k_t=1.0, K_d=50.0, n=2.0, mu_m=0.1, mu_p=0.2)
+    pop = sc.PopulationConfig(cell_num=2000, growth_rate=0.03, V_div=2.0, V_init=(0.8, 1.2), div_check_interval=5)
+    y = sc.simulate(net, sc.BinomialTauLeap(0.1), kin, "gpu", pop, T=200.0, readout="protein", seed=42)
+    assert y.shape == (2000, 1)
+    assert np.all(y >= 0)
+    assert y.mean() > 0.5 * 100.0 and np.isfinite(y.mean())
