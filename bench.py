@@ -14,8 +14,8 @@ timed region; outputs stay in HBM.  Prints one JSON line with
 Other workloads (--workload): g256 (precision comparison, 65,536 cells, 2000
 steps), batch (4096 uneven networks with population dynamics in one hipGraph),
 serve (Poisson arrivals at a fixed QPS), tauleap (binomial tau-leap on the G=20
-network), copy (plain device copy of known size, for scaling hardware byte
-counters).
+network), sbi (networks drawn from the prior, simulated in one plan), copy
+(plain device copy of known size, for scaling hardware byte counters).
 """
 from __future__ import annotations
 
@@ -259,12 +259,12 @@ def run_tauleap(a, sc) -> dict:
     if not a.no_cpu:
         from oracle import oracle as orc
         threads = max(1, min(16, os.cpu_count() or 1))
-        probe = 4 * threads
+        probe = 16 * threads
         t0 = time.perf_counter()
         orc.tauleap(net, kin, dt=dt, n_steps=steps, cell_num=cells, cell_lo=0, cell_hi=probe, seed=31,
                     threads=threads)
         rate = probe / max(time.perf_counter() - t0, 1e-6)
-        sample = int(min(cells, max(probe, rate * 10.0)))
+        sample = int(min(cells, max(probe, rate * 12.0)))
         sample = max(threads, (sample // threads) * threads)
         t0 = time.perf_counter()
         orc.tauleap(net, kin, dt=dt, n_steps=steps, cell_num=cells, cell_lo=0, cell_hi=sample, seed=31,
@@ -276,6 +276,28 @@ def run_tauleap(a, sc) -> dict:
                                          f"tauleap_binomial.jl, OpenMP over cells, {el_cpu:.1f} s"}
     plan.close()
     return out
+
+
+def run_sbi(a, sc) -> dict:
+    """Simulation-based-inference batch: draw networks from the native prior
+    (kinetics.jl:105-205), simulate them in one plan, capture counts."""
+    n, G, cells = a.networks if a.networks != 4096 else 1024, 20, a.cells or 1024
+    dt, T = 0.01, a.T or 50.0
+    t0 = time.perf_counter()
+    nets = sc.prior.sample_networks(G, n, seed=7)
+    t_prior = time.perf_counter() - t0
+    kin = sc.KineticParams(k_t=5.0, K_d=10.0, n=2.0, mu_m=1.0, mu_p=0.2)
+    cap = sc.CaptureModel(efficiency=0.1, efficiency_std=0.3, readout="mrna")
+    t0 = time.perf_counter()
+    plan = sc.Plan(nets, sc.CLE(dt), kin, cell_num=cells, T=T, readout="mrna", capture=cap, seed=11)
+    t_build = time.perf_counter() - t0
+    el, kms = timed_runs(plan, a.steps, a.warmup)
+    steps = sc.n_steps(T, dt)
+    cgs = float(n) * G * cells * steps
+    plan.close()
+    return {"workload": f"SBI batch: {n} prior networks (G={G}), {cells} cells each, {steps} steps, capture",
+            "networks": n, "prior_s": t_prior, "plan_build_s": t_build, "cgs_per_s": cgs * a.steps / el,
+            "kernel_ms": float(np.mean(kms)), "networks_per_s": n * a.steps / el}
 
 
 def run_batch(a, sc) -> dict:
@@ -372,7 +394,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["g20", "g256", "batch", "serve", "copy", "tauleap"], default="g20")
+    ap.add_argument("--workload", choices=["g20", "g256", "batch", "serve", "copy", "tauleap", "sbi"],
+                    default="g20")
     ap.add_argument("--cells", type=int, default=None, help="override cell count (testing only)")
     ap.add_argument("--tile", type=int, default=0, help="genes per lane for the FP32 stepper (2, 4, 8, 16)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -398,7 +421,7 @@ def main() -> int:
     if a.tile:
         sc.default_context().set_tile(1, min(256, 16 * a.tile), a.tile)
     runner = {"g20": run_g20, "g256": run_g256, "batch": run_batch, "serve": run_serve,
-              "tauleap": run_tauleap}[a.workload]
+              "tauleap": run_tauleap, "sbi": run_sbi}[a.workload]
     print(json.dumps(runner(a, sc)))
     return 0
 

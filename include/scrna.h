@@ -251,6 +251,47 @@ int scrna_cancel(scrna_server* srv, int64_t ticket);
 int scrna_server_stats(scrna_server* srv, scrna_latency* out);
 int scrna_server_destroy(scrna_server* srv);
 
+/* ---- Network prior and theta constructor (simulation-based inference) ----
+ * Host-only calls (no device work): they fill scrna_network arrays for
+ * scrna_simulate_cle_batch / scrna_plan_create. */
+
+/* Edge types of sample_network's `regulation` keyword (kinetics.jl:83-84). */
+typedef enum scrna_regulation {
+    SCRNA_REG_MIXED = 0,        /* additive + AND + OR, at least one non-additive edge */
+    SCRNA_REG_ADDITIVE = 1,
+    SCRNA_REG_COOPERATIVE = 2,  /* additive + AND */
+    SCRNA_REG_REDUNDANT = 3     /* additive + OR */
+} scrna_regulation;
+
+/* sample_network's keywords (kinetics.jl:105-112).  A range whose lo is < 0
+ * (or hi < lo) takes the G-dependent default of kinetics.jl:107-111. */
+typedef struct scrna_prior {
+    int32_t G;
+    int32_t regulation;                 /* scrna_regulation */
+    int32_t n_tf_lo, n_tf_hi;           /* n_tf_range */
+    int32_t targets_lo, targets_hi;     /* targets_per_tf_range */
+    int32_t coop_lo, coop_hi;           /* n_coop_range */
+    int32_t redun_lo, redun_hi;         /* n_redun_range */
+    int32_t infer_basals;               /* 1: LogNormal basals exp(0.5 z - 1); 0: exp(-1) */
+} scrna_prior;
+
+/* Capacity (gates) the output buffers of scrna_sample_network need. */
+int scrna_prior_max_gates(const scrna_prior* prior, int32_t* max_gates);
+/* Replaces sample_network(G; rng, ...) (kinetics.jl:105-181).  Network `index`
+ * of `seed` is a pure function of (seed, index): draws come from Philox4x32-10
+ * with counter (index, block, 0, 9), so batches can be generated in any order
+ * or in parallel.  Outputs: basals [G]; A [G*G] column-major (A[i + j*G] =
+ * effect of protein j on gene i, diagonal 0); gates: type (0 AND = CoopEdge,
+ * 1 OR = RedunEdge), target, 2 sources (ascending), strength; *n_gates set.
+ * All k_on / k_off of the sampled network are Inf (constitutive). */
+int scrna_sample_network(const scrna_prior* prior, uint64_t seed, uint32_t index, double* basals, double* A,
+                         int32_t* gate_type, int32_t* gate_target, int32_t* gate_src, double* gate_strength,
+                         int32_t max_gates, int32_t* n_gates);
+/* Replaces GeneNetwork(theta, G) (network.jl:233-256): n_theta = G(G-1)
+ * (basals exp(-1)) or G(G-1)+G (first G entries are log-basals); interactions
+ * in row-major order of the off-diagonal entries A[i,j]. */
+int scrna_network_from_theta(int32_t G, const double* theta, int64_t n_theta, double* basals, double* A);
+
 #ifdef __cplusplus
 }
 #endif

@@ -234,6 +234,54 @@ function gpu_apply_capture(X::Matrix{<:Real}, capture::CaptureModel;
     return Matrix{Int}(Y)
 end
 
+# ---- network prior (kinetics.jl:105-205) and theta constructor (network.jl:233-256) ----
+struct CPrior
+    G::Int32; regulation::Int32; n_tf_lo::Int32; n_tf_hi::Int32; targets_lo::Int32; targets_hi::Int32
+    coop_lo::Int32; coop_hi::Int32; redun_lo::Int32; redun_hi::Int32; infer_basals::Int32
+end
+
+const REGULATION = Dict(:mixed => Int32(0), :additive => Int32(1), :cooperative => Int32(2), :redundant => Int32(3))
+_rng(r) = r === nothing ? (Int32(-1), Int32(-1)) : (Int32(first(r)), Int32(last(r)))
+
+"""
+    sample_network_native(G; seed, index=0, regulation=:mixed, n_tf_range=nothing, ...)
+
+sample_network's prior (kinetics.jl:105-181) drawn natively; network `index`
+of `seed` is a pure function of the two (Philox stream), for SBI batches.
+"""
+function sample_network_native(G::Int; seed::UInt64, index::Integer=0, regulation::Symbol=:mixed,
+                               n_tf_range=nothing, targets_per_tf_range=nothing, n_coop_range=nothing,
+                               n_redun_range=nothing, infer_basals::Bool=true)
+    pr = Ref(CPrior(Int32(G), REGULATION[regulation], _rng(n_tf_range)..., _rng(targets_per_tf_range)...,
+                    _rng(n_coop_range)..., _rng(n_redun_range)..., Int32(infer_basals)))
+    mg = Ref{Int32}(0)
+    _check(ccall((:scrna_prior_max_gates, LIB), Cint, (Ref{CPrior}, Ref{Int32}), pr, mg), "sample_network")
+    m = Int(mg[])
+    basals = Vector{Float64}(undef, G); A = Matrix{Float64}(undef, G, G)
+    gtype = Vector{Int32}(undef, m); gtarget = Vector{Int32}(undef, m); gsrc = Vector{Int32}(undef, 2m)
+    gstr = Vector{Float64}(undef, m); ng = Ref{Int32}(0)
+    rc = ccall((:scrna_sample_network, LIB), Cint,
+               (Ref{CPrior}, UInt64, UInt32, Ptr{Float64}, Ptr{Float64}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32},
+                Ptr{Float64}, Int32, Ref{Int32}),
+               pr, seed, UInt32(index), basals, A, gtype, gtarget, gsrc, gstr, Int32(m), ng)
+    _check(rc, "sample_network")
+    coop = SyntheticscRNAseq.CoopEdge[]; redun = SyntheticscRNAseq.RedunEdge[]
+    for k in 1:ng[]
+        t = Int(gtarget[k]) + 1; src = [Int(gsrc[2k - 1]) + 1, Int(gsrc[2k]) + 1]
+        gtype[k] == 0 ? push!(coop, SyntheticscRNAseq.CoopEdge(t, src, gstr[k])) :
+                        push!(redun, SyntheticscRNAseq.RedunEdge(t, src, gstr[k]))
+    end
+    return GeneNetwork(G, basals, A, coop, redun, fill(Inf, G), fill(Inf, G))
+end
+
+"""GeneNetwork(theta, G) (network.jl:233-256), natively."""
+function network_from_theta_native(theta::Vector{Float64}, G::Int)
+    basals = Vector{Float64}(undef, G); A = Matrix{Float64}(undef, G, G)
+    _check(ccall((:scrna_network_from_theta, LIB), Cint, (Int32, Ptr{Float64}, Int64, Ptr{Float64}, Ptr{Float64}),
+                 Int32(G), theta, Int64(length(theta)), basals, A), "network_from_theta")
+    return GeneNetwork(G, basals, A)
+end
+
 function __init__()
     isfile(LIB) || @warn "libscrna.so not found at $LIB; set ENV[\"SCRNA_LIB\"]"
 end

@@ -112,6 +112,22 @@ class ScrnaRequest(C.Structure):
     ]
 
 
+class ScrnaPrior(C.Structure):
+    _fields_ = [
+        ("G", C.c_int32),
+        ("regulation", C.c_int32),
+        ("n_tf_lo", C.c_int32),
+        ("n_tf_hi", C.c_int32),
+        ("targets_lo", C.c_int32),
+        ("targets_hi", C.c_int32),
+        ("coop_lo", C.c_int32),
+        ("coop_hi", C.c_int32),
+        ("redun_lo", C.c_int32),
+        ("redun_hi", C.c_int32),
+        ("infer_basals", C.c_int32),
+    ]
+
+
 class ScrnaLatency(C.Structure):
     _fields_ = [
         ("p50_ms", C.c_double),
@@ -161,6 +177,10 @@ SIGNATURES = {
     "scrna_cancel": (C.c_int, [_VP, C.c_int64]),
     "scrna_server_stats": (C.c_int, [_VP, C.POINTER(ScrnaLatency)]),
     "scrna_server_destroy": (C.c_int, [_VP]),
+    "scrna_prior_max_gates": (C.c_int, [C.POINTER(ScrnaPrior), c_int32_p]),
+    "scrna_sample_network": (C.c_int, [C.POINTER(ScrnaPrior), C.c_uint64, C.c_uint32, c_double_p, c_double_p,
+                                       c_int32_p, c_int32_p, c_int32_p, c_double_p, C.c_int32, c_int32_p]),
+    "scrna_network_from_theta": (C.c_int, [C.c_int32, c_double_p, C.c_int64, c_double_p, c_double_p]),
 }
 
 _lib = None

@@ -52,6 +52,8 @@ STRUCTS = {
     "scrna_request": (_lib.ScrnaRequest, ["net", "kin", "run", "capture"]),
     "scrna_latency": (_lib.ScrnaLatency, ["p50_ms", "p99_ms", "mean_ms", "max_ms", "cell_gene_steps", "busy_s",
                                           "completed", "rejected", "cancelled"]),
+    "scrna_prior": (_lib.ScrnaPrior, ["G", "regulation", "n_tf_lo", "n_tf_hi", "targets_lo", "targets_hi",
+                                      "coop_lo", "coop_hi", "redun_lo", "redun_hi", "infer_basals"]),
 }
 
 
