@@ -292,6 +292,19 @@ int scrna_sample_network(const scrna_prior* prior, uint64_t seed, uint32_t index
  * in row-major order of the off-diagonal entries A[i,j]. */
 int scrna_network_from_theta(int32_t G, const double* theta, int64_t n_theta, double* basals, double* A);
 
+/* ---- Count-matrix writer ----
+ * Writes a cells x G int32 count matrix (column-major, element (c, g) at
+ * counts[g*ld + c], e.g. scrna_output.counts) as a 10x Genomics v2 directory,
+ * the format the reference's real-data comparison reads
+ * (experiments/compare_real_scrnaseq.jl:62-96): <dir>/matrix.mtx
+ * (MatrixMarket coordinate integer general, genes x cells, 1-based),
+ * <dir>/genes.tsv ("SYNG000001<TAB>name", names default to gene1..geneG) and
+ * <dir>/barcodes.tsv (16-nt barcodes of the cell index with "-1", after an
+ * optional prefix).  Creates <dir> if needed; *nnz_out (nullable) receives
+ * the number of nonzeros.  Host-only. */
+int scrna_write_10x(const char* dir, const int32_t* counts, int64_t cells, int32_t G, int64_t ld,
+                    const char* const* gene_names, const char* barcode_prefix, int64_t* nnz_out);
+
 #ifdef __cplusplus
 }
 #endif

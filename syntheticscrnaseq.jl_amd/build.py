@@ -21,7 +21,8 @@ OUT = os.path.join(HERE, "libscrna.so")
 OBJDIR = os.path.join(HERE, "_build")
 
 SOURCES = ["cle_kernels.hip", "runtime.cpp", "population.cpp", "server.cpp", "dense_kernels.hip",
-           "autotune.cpp", "unfused.hip", "tauleap_kernels.hip", "prior.cpp"]
+           "autotune.cpp", "unfused.hip", "tauleap_kernels.hip", "prior.cpp",
+           "mtx_writer.cpp"]
 # Stand-alone measurement programs (not linked into the library).
 PROBES = ["probe_valu_rates.hip"]
 

@@ -282,6 +282,28 @@ function network_from_theta_native(theta::Vector{Float64}, G::Int)
     return GeneNetwork(G, basals, A)
 end
 
+"""
+    write_10x(dir, counts; gene_names=nothing, barcode_prefix=nothing) -> nnz
+
+Write a cells x G count matrix as a 10x Genomics v2 directory (matrix.mtx,
+genes.tsv, barcodes.tsv), the layout experiments/compare_real_scrnaseq.jl:62-96 reads.
+"""
+function write_10x(dir::AbstractString, counts::AbstractMatrix{<:Integer}; gene_names=nothing,
+                   barcode_prefix=nothing)
+    Y = Matrix{Int32}(counts)
+    cells, G = size(Y)
+    names = gene_names === nothing ? C_NULL : [Base.unsafe_convert(Cstring, n) for n in gene_names]
+    nnz = Ref{Int64}(0)
+    GC.@preserve gene_names begin
+        rc = ccall((:scrna_write_10x, LIB), Cint,
+                   (Cstring, Ptr{Int32}, Int64, Int32, Int64, Ptr{Cstring}, Cstring, Ref{Int64}),
+                   dir, Y, Int64(cells), Int32(G), Int64(cells), names,
+                   barcode_prefix === nothing ? C_NULL : barcode_prefix, nnz)
+    end
+    _check(rc, "write_10x")
+    return nnz[]
+end
+
 function __init__()
     isfile(LIB) || @warn "libscrna.so not found at $LIB; set ENV[\"SCRNA_LIB\"]"
 end

@@ -6,7 +6,7 @@ PopulationConfig, CaptureModel, CLE, simulate, simulate_gpu_batch,
 apply_capture.  Every simulation runs in the HIP kernels of libscrna.so on a
 gfx950 device; nothing here computes a trajectory on the CPU.
 """
-from . import analytical, prior, samplers
+from . import analytical, io, prior, samplers
 from ._lib import BudgetError, Context, ScrnaError, UnsupportedError, default_context, load
 from .cle import CLE, BinomialTauLeap, Plan, SimResult, apply_capture, n_steps, simulate, simulate_gpu_batch
 from .kinetics import CaptureModel, KineticParams, PopulationConfig, steady_state

@@ -181,6 +181,8 @@ SIGNATURES = {
     "scrna_sample_network": (C.c_int, [C.POINTER(ScrnaPrior), C.c_uint64, C.c_uint32, c_double_p, c_double_p,
                                        c_int32_p, c_int32_p, c_int32_p, c_double_p, C.c_int32, c_int32_p]),
     "scrna_network_from_theta": (C.c_int, [C.c_int32, c_double_p, C.c_int64, c_double_p, c_double_p]),
+    "scrna_write_10x": (C.c_int, [C.c_char_p, c_int32_p, C.c_int64, C.c_int32, C.c_int64,
+                                  C.POINTER(C.c_char_p), C.c_char_p, C.POINTER(C.c_int64)]),
 }
 
 _lib = None
