@@ -678,7 +678,7 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
             lg.lds = tl ? tauleap_lds_bytes(S, lg.nbuf, lg.NW, lg.CH) : stepper_lds_bytes(S, lg.nbuf);
             const int per_cu = tl ? tauleap_blocks_per_cu(lg.NW, lg.lds)
                                   : stepper_blocks_per_cu(lg.CH, lg.NW, lg.lds);
-            lg.nseg = choose_segments(lg.n_tiles, (long long)per_cu * ctx->num_cu, group_steps[kv.first]);
+            lg.nseg = choose_segments(lg.n_tiles, (long long)per_cu * ctx->num_cu, group_steps[kv.first], tl ? 8 : 16);
             if (lg.nseg > 1) {
                 lg.sched_off = P->sched_words;
                 P->sched_words += sched_words(lg.n_tiles);
@@ -832,24 +832,23 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
     return SCRNA_OK;
 }
 
-// Segments per tile for a whole-horizon FP32 stepper launch.  One workgroup
-// per tile runs in ceil(n_tiles / resident) rounds of a full horizon, and the
-// last round is often nearly empty; k segments per tile cut the quantum to
-// 1/k of the horizon.  Pick k in 1..8 minimising ceil(n_tiles*k/resident)/k
-// (smallest k on ties), with at least 64 steps per segment.
-int choose_segments(int n_tiles, long long resident, int64_t steps) {
-    if (resident <= 0 || n_tiles <= resident) return 1;
-    int best = 1;
-    double best_cost = (double)((n_tiles + resident - 1) / resident);
-    for (int k = 2; k <= 8 && steps / k >= 64; ++k) {
-        const long long work = (long long)n_tiles * k;
-        const double cost = (double)((work + resident - 1) / resident) / k;
-        if (cost < best_cost - 1e-9) {
-            best = k;
-            best_cost = cost;
-        }
-    }
-    return best;
+// Segments per tile for a whole-horizon stepper launch.  With one workgroup
+// per tile the device runs ceil(n_tiles / resident) rounds of a full horizon:
+// the last round is often nearly empty, and even a single full round leaves
+// SIMDs idle because a workgroup's waves load the four SIMDs unevenly and
+// nothing refills them.  k segments per tile cut the work quantum to 1/k of
+// the horizon and let the dispatcher refill freed slots: the makespan is
+// about n_tiles/resident + 1/k full horizons.  Measured on the G=20 benchmark
+// network (1536 resident workgroups): 1563 tiles took 33.8 ms with k = 8 and
+// 33.2 ms with k = 16, while 1536 tiles in one unsegmented round took 37.1 ms.  So:
+// segment whenever the tiles fill at least half the device, with the largest
+// k <= max_k that keeps >= 64 steps per segment.  Below that, segments of a
+// tile would only run one after another.  The tau-leap stepper (one 7-wave
+// workgroup per CU) measured best at max_k = 8 (331 ms) rather than 16 (344 ms).
+int choose_segments(int n_tiles, long long resident, int64_t steps, int max_k) {
+    if (resident <= 0 || 2LL * n_tiles < resident) return 1;
+    const int64_t k = std::min<int64_t>(max_k, steps / 64);
+    return k < 1 ? 1 : (int)k;
 }
 
 // Run steps [s_begin, s_end) of every network (clamped to each horizon).

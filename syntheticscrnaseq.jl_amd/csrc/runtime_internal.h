@@ -139,7 +139,7 @@ int plan_download(scrna_plan* P, int32_t i, scrna_output* out);
 void plan_release(scrna_plan* P);
 int population_launch(scrna_plan* P, hipStream_t st);
 int launch_steps(scrna_plan* P, hipStream_t st, int s_begin, int s_end, bool segmented);
-int choose_segments(int n_tiles, long long resident, int64_t steps);
+int choose_segments(int n_tiles, long long resident, int64_t steps, int max_k);
 int plan_make_graph(scrna_plan* P);
 int unfused_prepare(scrna_plan* P, UnfusedNet& u);
 int unfused_launch(scrna_plan* P, UnfusedNet& u, hipStream_t st);

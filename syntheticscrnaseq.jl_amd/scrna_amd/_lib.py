@@ -11,7 +11,8 @@ import os
 import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "libscrna.so")
+# SCRNA_LIB overrides the in-tree library (as in the Julia shim), e.g. to compare builds.
+LIB_PATH = os.environ.get("SCRNA_LIB") or os.path.join(os.path.dirname(_HERE), "libscrna.so")
 
 SCRNA_OK = 0
 SCRNA_ERR_INVALID = -1
