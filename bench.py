@@ -55,14 +55,18 @@ def valu_census(net, dense: bool = False) -> dict:
     """Algorithmic VALU work per cell*gene*step of a network (fixed operation
     counts), priced in FP32-FMA issue units with the measured gfx950 costs.
 
-    Operation counts:
+    Operation counts (the fewest per-lane instructions the algorithm needs on
+    gfx950, so that the fraction of peak is a speed-of-light fraction):
       update E = 36 ops (cle.jl:79-120, every +,-,*,/,max,sqrt,pow = 1), of
         which 2 sqrt and 1 reciprocal are transcendental and 33 FP32;
       regulation R = 2 * additive in-edges per gene (0 on the MFMA path, whose
         contraction is MFMA work) + 3 * gate source incidences per gene, FP32;
-      Box-Muller for the pair of normals: log, sqrt, sin, cos + 5 FP32;
-      Philox4x32-10: 20 32x32-bit products + 58 xor/add per call, 0.5 call per
-        gene (+0.25 with telegraph switching, which adds 3 integer ops).
+      Box-Muller for the pair of normals: log, sin, cos + 5 FP32 (its radius
+        square root folds into the update's two, sqrt(r^2 * var));
+      Philox4x32-10: per round two 32x32-bit products and two 3-input xors
+        (v_bitop3_b32); the key schedule is wave-uniform, scalar work.  So 20
+        products + 20 integer ops per call, 0.5 call per gene (+0.25 with
+        telegraph switching, which adds 3 integer ops).
     Returns the class counts, the FMA-equivalent total and the FP32-equivalent
     flops (2 per FMA-equivalent op).
     """
@@ -73,8 +77,8 @@ def valu_census(net, dense: bool = False) -> dict:
     inc = sum(len(e.sources) for e in list(net.cooperative) + list(net.redundant)) / G
     tele = bool(np.any(np.isfinite(net.k_on)))
     calls = 0.5 + (0.25 if tele else 0.0)
-    counts = {"fp32": 33 + 2 * d_add + 3 * inc + 5, "trans": 3 + 4,
-              "int": 58 * calls + (3 if tele else 0), "mul32": 20 * calls}
+    counts = {"fp32": 33 + 2 * d_add + 3 * inc + 5, "trans": 3 + 3,
+              "int": 20 * calls + (3 if tele else 0), "mul32": 20 * calls}
     fma_eq = (counts["fp32"] * COST_FP32 + counts["trans"] * COST_TRANS + counts["int"] * COST_INT
               + counts["mul32"] * COST_MUL32)
     return {"counts": {k: round(v, 3) for k, v in counts.items()}, "fma_equiv_ops": fma_eq,

@@ -64,12 +64,12 @@ static inline void orc_draw(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2
 }
 
 /* Box-Muller on two u32: radius uniform has 23+1 exact bits in (0,1), angle
- * 24 exact bits in revolutions [0,1).  Both are exactly representable in
+ * 23 exact bits in revolutions [0,1).  Both are exactly representable in
  * float, so the GPU's float transform and this double one see identical
  * uniforms. */
 void orc_normal_pair(uint32_t a, uint32_t b, double* z0, double* z1) {
     double u1 = ((double)(a >> 9) + 0.5) * (1.0 / 8388608.0);   /* 2^-23 */
-    double th = (double)(b >> 8) * (1.0 / 16777216.0);          /* 2^-24 */
+    double th = (double)(b >> 9) * (1.0 / 8388608.0);           /* 2^-23 */
     double r = sqrt(-2.0 * log(u1));
     double ang = 2.0 * M_PI * th;
     *z0 = r * cos(ang);

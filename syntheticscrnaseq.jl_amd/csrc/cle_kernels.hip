@@ -1,6 +1,6 @@
 // gfx950 kernels for the Chemical Langevin Equation path.
 //
-//   cle_stepper_kernel<CH>  persistent multi-step Euler-Maruyama CLE
+//   cle_stepper_kernel<CH,INJ,POP>  persistent multi-step Euler-Maruyama CLE
 //                           (reference update rule: src/algorithms/cle.jl:78-121;
 //                           GPU form ext/SyntheticscRNAseqCUDAExt.jl:87-108)
 //   capture_kernel          LogNormal-Binomial capture (src/capture.jl:25-47)
@@ -13,9 +13,11 @@
 // owner of a "source" gene writes act = x^n/(1+x^n) and rep = 1/(1+x^n)
 // (x = p/K) into LDS rows [2*slot][64 lanes] and [2*slot+1][64 lanes]; after
 // one workgroup barrier each wave gathers its genes' in-edges from LDS.
-// Edge lists are uniform per wave (every lane of a wave works on the same
-// gene), so they stream through the scalar cache.  Only snapshots and the
-// final state touch HBM.
+// Everything about a wave's genes is uniform across the wave (every lane
+// works on the same genes): switch thresholds, basal rates, the Hill rows they
+// write and their first in-edges are set up once in VGPRs; further edges and
+// gates stream through the scalar cache.  Only snapshots and the final state
+// touch HBM.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -51,32 +53,48 @@ __device__ __forceinline__ void hill_pair(float p, float invK, float hill_n, flo
     act = xn * rep;
 }
 
-// Hill values of a wave's CH genes that are regulation sources -> LDS rows
-// [2*slot] (act) and [2*slot+1] (rep), one column per lane.
+// Hill values of a wave's source genes (bit k of srcm: gene g0+k) -> the LDS
+// row pair at byte address haddr[k] + hbase: act in the first row, rep in the
+// second (64 floats further), one column per lane.
 template <int CH, bool HILL2>
-__device__ __forceinline__ void hill_to_lds(const float (&p)[CH], int g0, int G, const int32_t* __restrict__ slot_of,
-                                            float invK, float hill_n, float* __restrict__ H, int lane) {
+__device__ __forceinline__ void hill_to_lds(const float (&p)[CH], uint32_t srcm, const uint32_t (&haddr)[CH],
+                                            uint32_t hbase, float invK, float hill_n, float* __restrict__ lds) {
 #pragma unroll
     for (int k = 0; k < CH; ++k) {
-        const int g = g0 + k;
-        if (g < G) {
-            const int slot = slot_of[g];
-            if (slot >= 0) {
-                float act, rep;
-                hill_pair<HILL2>(p[k], invK, hill_n, act, rep);
-                H[(2 * slot) * 64 + lane] = act;
-                H[(2 * slot + 1) * 64 + lane] = rep;
-            }
+        if ((srcm >> k) & 1u) {
+            float act, rep;
+            hill_pair<HILL2>(p[k], invK, hill_n, act, rep);
+            float* row = (float*)((char*)lds + (haddr[k] + hbase));
+            row[0] = act;
+            row[64] = rep;
         }
     }
 }
 
-// Up to 4 genes per lane the kernel fits 8 waves per SIMD (the SGPR budget is
-// the limit); asking for that occupancy measured 10% faster on the G=20
-// benchmark (33.5 vs 37.1 ms).  Wider tiles keep the compiler's choice, since
-// their registers would spill.
-template <int CH>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(CH <= 4 ? 8 : 1)))
+// Occupancy and register slots for up to 4 genes per lane.  The kernel asks
+// for 7 waves per SIMD (72 VGPRs, 94 SGPRs), which holds two register slots
+// per gene for in-edges without spilling.  On the G=20 benchmark network
+// (kernel ms): 2 slots at 7 waves 23.6, 2 slots at 8 waves (VGPRs spill) 23.7,
+// 1 slot at 8 waves 24.4, 1 slot at 7 waves 24.6.  Wider tiles keep the
+// compiler's choice of occupancy and use no register slots, since their
+// registers would spill.
+constexpr int kEdgeSlots4 = 2;
+constexpr int kWavesPerSimd4 = 7;
+//
+// INJ and POP say whether a launch may hold networks with injected noise or a
+// population; the flags in each blob still decide per network.  Launches
+// without them (the common case) get a kernel without those paths, whose
+// pointers, offsets and flags would otherwise occupy scalar registers for the
+// whole horizon.
+//
+// Scalar-register economy.  The step loop is long and every uniform value the
+// compiler keeps live across it takes an SGPR (a uniform boolean takes two, as
+// a lane mask); past the budget they spill into VGPR lanes and each reload is
+// a VALU v_readlane.  So the network flags and the switch mask are re-read as
+// opaque values at the top of every step: their per-gene tests are then
+// recomputed on the scalar unit instead of being hoisted as lane masks.
+template <int CH, bool INJ, bool POP>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(CH <= 4 ? kWavesPerSimd4 : 1)))
 cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ tiles,
                    float* __restrict__ state_pool, float* __restrict__ snap_pool,
                    uint8_t* __restrict__ prom_pool, double* __restrict__ vol_pool,
@@ -86,7 +104,8 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
     __shared__ int sh_ticket, sh_fail;
     const SegmentSlot sl = segment_take(sched, n_tiles, &sh_ticket);
     const Tile tile = tiles[sl.tile];
-    const uint32_t* __restrict__ B = blobs + tile.blob;
+    const uint32_t blob_off = __builtin_amdgcn_readfirstlane(tile.blob);
+    const uint32_t* __restrict__ B = blobs + blob_off;
     const int G = (int)B[H_G];
     const int n_steps = (int)B[H_NSTEPS];
     // Step range of this launch, clamped to the tile and to the network's
@@ -97,39 +116,18 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
     segment_range(r_lo, r_hi, sl.seg, n_seg, &s_start, &s_stop);
     if (!segment_wait(sched, sl, wait_ticks, &sh_fail)) return;
     const bool resume = s_start > 0;       // later chunks and segments continue from the state pool
-    const int S = (int)B[H_S];
+    // Values used before and after the step loop only; the loop re-reads
+    // what it needs from the blob.
     const uint32_t flags = B[H_FLAGS];
     const int snap_every = (int)B[H_SNAP_EVERY];
     const int C = (int)B[H_CELLS];
-    const uint32_t k0 = B[H_SEED_LO], k1 = B[H_SEED_HI];
-    const uint32_t c3n = (B[H_NETID] << 4) | kNormal;
-    const uint32_t c3s = (B[H_NETID] << 4) | kSwitch;
-    const float dt = as_f(B[H_DT]);
-    const float neg2ln2dt = as_f(B[H_NEG2LN2DT]);
-    const float k_t = as_f(B[H_KT]);
-    const float mu_m = as_f(B[H_MUM]);
-    const float mu_p = as_f(B[H_MUP]);
-    const float invK = as_f(B[H_INVK]);
-    const float hill_n = as_f(B[H_HILLN]);
-    const float sqrt_dt = as_f(B[H_SQRTDT]);
-    const double growth = __longlong_as_double((long long)u64_at(B, H_GROWTH_LO));
-    const bool hill2 = (flags & F_HILL2) != 0;
     const bool tele = (flags & F_TELE) != 0;
-    const bool inject = (flags & F_INJECT) != 0;
-    const bool pop = (flags & F_POP) != 0;
-    const uint32_t* __restrict__ beta_w = B + B[H_OFF_BETA];
-    const int32_t* __restrict__ slot_of = (const int32_t*)(B + B[H_OFF_SLOT]);
-    const int32_t* __restrict__ in_off = (const int32_t*)(B + B[H_OFF_INOFF]);
-    const uint32_t* __restrict__ edges = B + B[H_OFF_EDGES];
-    const int32_t* __restrict__ gate_off = (const int32_t*)(B + B[H_OFF_GATEOFF]);
-    const uint32_t* __restrict__ gates = B + B[H_OFF_GATES];
+    const bool pop = POP && (flags & F_POP) != 0;
     const uint32_t* __restrict__ telew = B + B[H_OFF_TELE];
     const uint32_t* __restrict__ initw = B + B[H_OFF_INIT];
     const size_t off_state = (size_t)u64_at(B, H_OUT_FINAL_LO);
-    const size_t off_snap = (size_t)u64_at(B, H_OUT_SNAP_LO);
     const size_t off_prom = (size_t)u64_at(B, H_OUT_PROM_LO);
     const size_t off_vol = (size_t)u64_at(B, H_OUT_V_LO);
-    const size_t off_inj = (size_t)u64_at(B, H_INJ_LO);
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -138,157 +136,290 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
     const int g0 = __builtin_amdgcn_readfirstlane(wave * CH);
 
     float m[CH], p[CH];
-    uint32_t on = 0;                                       // promoter bit k = gene g0+k
+    // Promoter states as wave lane masks: bit `lane` of on[k] is the state of
+    // gene g0+k in that lane's cell.  The switch tests then produce masks
+    // straight from v_cmp and combine them on the scalar unit, and the gating
+    // of transcription is one v_cndmask on the mask.
+    uint64_t on[CH];
+    // Per-gene constants of the step loop kept in VGPRs (uniform values; VALU
+    // operands at no cost, and no scalar registers or reloads across the
+    // loop): the low words of the OFF->ON and ON->OFF thresholds (0 for genes
+    // that never switch, which then never flip) and the LDS byte address of the
+    // gene's Hill row pair in this lane (source genes only).
+    uint32_t thr_on[CH], thr_off[CH], haddr[CH];
+    float beta_v[CH];                                      // basal rate of each gene
+    // The first EK additive in-edges of each gene as (LDS byte address in this
+    // lane's column, weight); unused slots read the buffer's zero row with
+    // weight 0.  Further edges and gates go through the blob (bits of
+    // extra_mask: 0..15 more edges, 16..31 gates).
+    constexpr int EK = CH <= 4 ? kEdgeSlots4 : 0;
+    uint32_t eaddr[CH][EK > 0 ? EK : 1];
+    float ew[CH][EK > 0 ? EK : 1];
     uint32_t sw_mask = 0;                                  // bit k: gene g0+k switches
+    uint32_t hi_mask = 0;                                  // bit 2k / 2k+1: ON / OFF threshold is 2^32
+    uint32_t src_mask = 0;                                 // bit k: gene g0+k is a regulation source
+    uint32_t extra_mask = 0;
     double V = 1.0;                                       // cell volume (population runs)
+    const int S = (int)B[H_S];
+    const uint32_t zaddr = (uint32_t)((2 * S * 64 + lane) * 4);   // zero row of a buffer, this lane
+    {
+        const int32_t* __restrict__ slot_of = (const int32_t*)(B + B[H_OFF_SLOT]);
+        const uint32_t* __restrict__ beta_w = B + B[H_OFF_BETA];
+        const int32_t* __restrict__ in_off = (const int32_t*)(B + B[H_OFF_INOFF]);
+        const uint32_t* __restrict__ edges = B + B[H_OFF_EDGES];
+        const int32_t* __restrict__ gate_off = (const int32_t*)(B + B[H_OFF_GATEOFF]);
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+            const int g = g0 + k;
+            thr_on[k] = thr_off[k] = haddr[k] = 0u;
+            beta_v[k] = 0.0f;
+#pragma unroll
+            for (int i = 0; i < EK; ++i) {
+                eaddr[k][i] = zaddr;
+                ew[k][i] = 0.0f;
+            }
+            if (g < G) {
+                beta_v[k] = as_f(beta_w[g]);
+                const int e0 = in_off[g], e1 = in_off[g + 1];
+#pragma unroll
+                for (int i = 0; i < EK; ++i) {
+                    if (e0 + i < e1) {
+                        eaddr[k][i] = (uint32_t)(((int)edges[2 * (e0 + i)] * 64 + lane) * 4);
+                        ew[k][i] = as_f(edges[2 * (e0 + i) + 1]);
+                    }
+                }
+                if (e1 - e0 > EK) extra_mask |= 1u << k;
+                if ((flags & F_GATES) && gate_off[g + 1] > gate_off[g]) extra_mask |= 1u << (16 + k);
+                const uint32_t* t = telew + g * 8;
+                if (tele && (t[T_FLAGS] & 1u)) {
+                    thr_on[k] = t[T_ON_LO];
+                    thr_off[k] = t[T_OFF_LO];
+                    hi_mask |= ((t[T_ON_HI] != 0u) ? 1u : 0u) << (2 * k);
+                    hi_mask |= ((t[T_OFF_HI] != 0u) ? 1u : 0u) << (2 * k + 1);
+                }
+                const int slot = slot_of[g];
+                if (slot >= 0) {
+                    src_mask |= 1u << k;
+                    haddr[k] = (uint32_t)((2 * slot * 64 + lane) * 4);
+                }
+            }
+            asm volatile("" : "+v"(thr_on[k]), "+v"(thr_off[k]), "+v"(haddr[k]), "+v"(beta_v[k]));
+#pragma unroll
+            for (int i = 0; i < EK; ++i) asm volatile("" : "+v"(eaddr[k][i]), "+v"(ew[k][i]));
+        }
+    }
+    // The zero rows (one per buffer) are never written by the steps.
+    if (wave == 0) {
+        lds[2 * S * 64 + lane] = 0.0f;
+        if (nbuf == 2) lds[(2 * S + 1 + 2 * S) * 64 + lane] = 0.0f;
+    }
 #pragma unroll
     for (int k = 0; k < CH; ++k) {
         const int g = g0 + k;
         m[k] = 0.0f;
         p[k] = 0.0f;
+        on[k] = ~0ull;
         if (g < G) {
             const uint32_t tf = telew[g * 8 + T_FLAGS];
             sw_mask |= (tf & 1u) << k;
+            bool o = true;   // constitutive genes are always ON
             if (resume) {
                 m[k] = state_pool[off_state + (size_t)g * C + cell];
                 p[k] = state_pool[off_state + (size_t)(G + g) * C + cell];
-                on |= (uint32_t)(prom_pool[off_prom + (size_t)g * C + cell] != 0) << k;
+                o = prom_pool[off_prom + (size_t)g * C + cell] != 0;
             } else {
                 m[k] = as_f(initw[2 * g]);
                 p[k] = as_f(initw[2 * g + 1]);
-                bool o = true;   // constitutive genes are always ON
                 if (tele && (tf & 2u)) {
                     const U4 u = philox4x32_10((uint32_t)cell, 0u, (uint32_t)(g >> 2),
-                                               (B[H_NETID] << 4) | kPromInit, k0, k1);
+                                               (B[H_NETID] << 4) | kPromInit, B[H_SEED_LO], B[H_SEED_HI]);
                     const uint32_t uu[4] = {u.x, u.y, u.z, u.w};
                     o = (uint64_t)uu[g & 3] < u64_at(telew, g * 8 + T_INIT_LO);
                 }
-                on |= (uint32_t)o << k;
             }
+            on[k] = __builtin_amdgcn_ballot_w64(o);
         }
     }
     if (pop) V = vol_pool[off_vol + cell];   // written by the population init kernel or a previous phase
-    if (!tele) sw_mask = 0;
-    const bool any_switch = sw_mask != 0;
+    sw_mask = tele ? __builtin_amdgcn_readfirstlane(sw_mask) : 0u;
+    hi_mask = __builtin_amdgcn_readfirstlane(hi_mask);
+    src_mask = __builtin_amdgcn_readfirstlane(src_mask);
+    extra_mask = __builtin_amdgcn_readfirstlane(extra_mask);
 
-    const int rows = 2 * S;
+    // Next step after which a snapshot is due (a countdown instead of a
+    // per-step division).
+    int next_snap = snap_every > 0 ? (s_start / snap_every + 1) * snap_every - 1 : INT32_MAX;
     for (int s = s_start; s < s_stop; ++s) {
-        float* __restrict__ H = lds + (nbuf == 2 ? (s & 1) * rows * 64 : 0);
+        // The blob offset is made opaque at every step, so the header words
+        // and per-gene addresses below are re-read through the scalar cache
+        // inside the step instead of being kept live (and spilled) across it.
+        // The asm is not volatile (a volatile one would count as a memory
+        // write and turn every later blob read into a vector load) and takes s
+        // as an input so that it stays inside the loop; it works on the
+        // offset, not on the pointer, which keeps the address space global.
+        uint32_t boff = blob_off, swm = sw_mask, him = hi_mask, srcm = src_mask, xtra = extra_mask;
+        asm("" : "+s"(boff) : "s"(s));
+        asm("" : "+s"(swm) : "s"(s));
+        asm("" : "+s"(him) : "s"(s));
+        asm("" : "+s"(srcm) : "s"(s));
+        asm("" : "+s"(xtra) : "s"(s));
+        const uint32_t* __restrict__ Bs = blobs + boff;
+        const uint32_t fl = Bs[H_FLAGS];
+        const int nv = min(CH, (int)Bs[H_G] - g0);            // genes of this wave
+        const uint32_t sk0 = Bs[H_SEED_LO], sk1 = Bs[H_SEED_HI];
+        const uint32_t netc = Bs[H_NETID] << 4;
+        // Buffer of this step: 2S Hill rows and a zero row.
+        float* __restrict__ H = lds + (nbuf == 2 ? (s & 1) * (2 * (int)Bs[H_S] + 1) * 64 : 0);
+        const uint32_t hbase = (uint32_t)((H - lds) * 4);
 
-        // Telegraph switching first (tauleap_binomial.jl:110-121).
-        if (any_switch) {
+        // Telegraph switching first (tauleap_binomial.jl:110-121).  A lane
+        // flips when its u32 is below the threshold of its current state
+        // (ON cells use the ON->OFF one); thresholds are at most 2^32, and one
+        // of 2^32 (hi_mask) flips every lane.  Genes that never switch have
+        // zero thresholds and never flip.
+        if (swm != 0u) {
 #pragma unroll
             for (int k = 0; k < CH; k += 4) {
                 const int g = g0 + k;
-                if (g < G) {
-                    const U4 u = philox4x32_10_uk((uint32_t)cell, (uint32_t)s, (uint32_t)(g >> 2), c3s, k0, k1);
+                if (k < nv) {
+                    const U4 u = philox4x32_10((uint32_t)cell, (uint32_t)s, (uint32_t)(g >> 2), netc | kSwitch, sk0,
+                                               sk1);
                     const uint32_t uu[4] = {u.x, u.y, u.z, u.w};
 #pragma unroll
                     for (int j = 0; j < 4 && k + j < CH; ++j) {
-                        if ((sw_mask >> (k + j)) & 1u) {
-                            // Both thresholds are uniform scalar loads; the lane's promoter
-                            // state picks one (readfirstlane keeps the selection on values,
-                            // not on a per-lane address).  thr <= 2^32, so x < thr is
-                            // hi != 0 or x < lo.
-                            const uint32_t* t = telew + (g + j) * 8;
-                            const uint32_t on_lo = __builtin_amdgcn_readfirstlane(t[T_ON_LO]);
-                            const uint32_t on_hi = __builtin_amdgcn_readfirstlane(t[T_ON_HI]);
-                            const uint32_t off_lo = __builtin_amdgcn_readfirstlane(t[T_OFF_LO]);
-                            const uint32_t off_hi = __builtin_amdgcn_readfirstlane(t[T_OFF_HI]);
-                            const uint32_t x = uu[(g & 3) + j];
-                            const bool cur = (on >> (k + j)) & 1u;
-                            const uint32_t lo = cur ? off_lo : on_lo;
-                            const uint32_t hi = cur ? off_hi : on_hi;
-                            const bool flip = (hi != 0u) | (x < lo);
-                            on ^= (uint32_t)flip << (k + j);
+                        const uint32_t x = uu[(g & 3) + j];
+                        const uint32_t thr =
+                            __builtin_amdgcn_inverse_ballot_w64(on[k + j]) ? thr_off[k + j] : thr_on[k + j];
+                        uint64_t flip = __builtin_amdgcn_ballot_w64(x < thr);
+                        if (him != 0u) {
+                            const uint64_t all_on = ((him >> (2 * (k + j))) & 1u) ? ~0ull : 0ull;
+                            const uint64_t all_off = ((him >> (2 * (k + j) + 1)) & 1u) ? ~0ull : 0ull;
+                            flip |= (on[k + j] & all_off) | (~on[k + j] & all_on);
                         }
+                        on[k + j] ^= flip;
                     }
                 }
             }
         }
 
         // Hill values of this wave's source genes -> LDS (cle.jl:79-82).
-        if (hill2) {
-            hill_to_lds<CH, true>(p, g0, G, slot_of, invK, hill_n, H, lane);
-        } else {
-            hill_to_lds<CH, false>(p, g0, G, slot_of, invK, hill_n, H, lane);
+        {
+            const float invK = as_f(Bs[H_INVK]);
+            if (fl & F_HILL2)
+                hill_to_lds<CH, true>(p, srcm, haddr, hbase, invK, 2.0f, lds);
+            else
+                hill_to_lds<CH, false>(p, srcm, haddr, hbase, invK, as_f(Bs[H_HILLN]), lds);
         }
         __syncthreads();
 
         // Regulation + Euler-Maruyama update, two genes per Philox block.
+        {
+            const float dt = as_f(Bs[H_DT]);
+            const float neg2ln2dt = as_f(Bs[H_NEG2LN2DT]);
+            const float k_t = as_f(Bs[H_KT]);
+            const float mu_m = as_f(Bs[H_MUM]);
+            const float mu_p = as_f(Bs[H_MUP]);
+            const int32_t* __restrict__ in_off = (const int32_t*)(Bs + Bs[H_OFF_INOFF]);
+            const uint32_t* __restrict__ edges = Bs + Bs[H_OFF_EDGES];
+            const int32_t* __restrict__ gate_off = (const int32_t*)(Bs + Bs[H_OFF_GATEOFF]);
+            const uint32_t* __restrict__ gates = Bs + Bs[H_OFF_GATES];
 #pragma unroll
-        for (int k = 0; k < CH; k += 2) {
-            const int g = g0 + k;
-            if (g < G) {
-                float xm[2], xp[2];
-                if (inject) {
+            for (int k = 0; k < CH; k += 2) {
+                const int g = g0 + k;
+                if (k < nv) {
+                    // Noise of gene j: xi_m = c[j]*sqrt(r2[j]), xi_p = sn[j]*sqrt(r2[j]) (Box-Muller
+                    // radius squared, scaled by dt, and the two unit directions), so that
+                    // xi*sqrt(var) below costs one square root, sqrt(r2*var).
+                    float r2[2], c[2], sn[2];
+                    if (INJ && (fl & F_INJECT)) {
+                        const int Gs = (int)Bs[H_G];
+                        const size_t Cs = (size_t)Bs[H_CELLS];
+                        const float* __restrict__ inj = inj_pool + (size_t)u64_at(Bs, H_INJ_LO);
 #pragma unroll
-                    for (int j = 0; j < 2; ++j) {
-                        const int gg = (g + j < G) ? g + j : g;
-                        const size_t base = ((size_t)s * G + gg) * 2;
-                        xm[j] = inj_pool[off_inj + base * C + cell] * sqrt_dt;
-                        xp[j] = inj_pool[off_inj + (base + 1) * C + cell] * sqrt_dt;
-                    }
-                } else {
-                    const U4 u = philox4x32_10_uk((uint32_t)cell, (uint32_t)s, (uint32_t)(g >> 1), c3n, k0, k1);
-                    normal_pair_scaled(u.x, u.y, neg2ln2dt, xm[0], xp[0]);
-                    normal_pair_scaled(u.z, u.w, neg2ln2dt, xm[1], xp[1]);
-                }
-#pragma unroll
-                for (int j = 0; j < 2 && k + j < CH; ++j) {
-                    const int gg = g + j;
-                    if (gg < G) {
-                        // Additive Hill regulation (cle.jl:84-86).
-                        float reg = 0.0f;
-                        const int e1 = in_off[gg + 1];
-                        for (int e = in_off[gg]; e < e1; ++e)
-                            reg = fmaf(as_f(edges[2 * e + 1]), H[edges[2 * e] * 64 + lane], reg);
-                        // AND / OR gates (cle.jl:89-106); OR uses rep = 1 - act.
-                        if (flags & F_GATES) {
-                            const int q1 = gate_off[gg + 1];
-                            for (int q = gate_off[gg]; q < q1;) {
-                                const uint32_t hd = gates[q];
-                                const uint32_t type = hd >> 24;
-                                const int ns = (int)(hd & 0xffffu);
-                                float prod = 1.0f;
-                                for (int i = 0; i < ns; ++i)
-                                    prod *= H[(2 * (int)gates[q + 2 + i] + (int)type) * 64 + lane];
-                                reg = fmaf(as_f(gates[q + 1]), type == GATE_AND ? prod : 1.0f - prod, reg);
-                                q += 2 + ns;
-                            }
+                        for (int j = 0; j < 2; ++j) {
+                            const int gg = (k + j < nv) ? g + j : g;
+                            const size_t base = ((size_t)s * Gs + gg) * 2;
+                            r2[j] = dt;
+                            c[j] = inj[base * Cs + cell];
+                            sn[j] = inj[(base + 1) * Cs + cell];
                         }
-                        const float beta = as_f(beta_w[gg]);
-                        const bool o = (on >> (k + j)) & 1u;
-                        // telegraph gating of the transcription propensity
-                        // (tauleap_binomial.jl:134-137); volume scaling (cle.jl:224-225)
-                        const float tx = o ? (reg + beta) * (float)V : 0.0f;
-                        // mRNA (cle.jl:110-113)
-                        const float mm = m[k + j];
-                        const float var_m = fmaxf(fmaf(mu_m, fmaxf(mm, 0.0f), tx), 0.0f);
-                        float mn = fmaf(fmaf(-mu_m, mm, tx), dt, mm);
-                        mn = fmaxf(fmaf(xm[j], __builtin_amdgcn_sqrtf(var_m), mn), 0.0f);
-                        // protein with the updated mRNA (cle.jl:116-120)
-                        const float pp = p[k + j];
-                        const float var_p = fmaxf(fmaf(k_t, mn, mu_p * fmaxf(pp, 0.0f)), 0.0f);
-                        float pn = fmaf(fmaf(k_t, mn, -mu_p * pp), dt, pp);
-                        pn = fmaxf(fmaf(xp[j], __builtin_amdgcn_sqrtf(var_p), pn), 0.0f);
-                        m[k + j] = mn;
-                        p[k + j] = pn;
+                    } else {
+                        const U4 u = philox4x32_10((uint32_t)cell, (uint32_t)s, (uint32_t)(g >> 1), netc | kNormal,
+                                                   sk0, sk1);
+                        normal_pair_r2(u.x, u.y, neg2ln2dt, r2[0], c[0], sn[0]);
+                        normal_pair_r2(u.z, u.w, neg2ln2dt, r2[1], c[1], sn[1]);
+                    }
+#pragma unroll
+                    for (int j = 0; j < 2 && k + j < CH; ++j) {
+                        const int gg = g + j;
+                        if (k + j < nv) {
+                            // Additive Hill regulation (cle.jl:84-86): the register slots,
+                            // then any further (LDS row, weight) pairs of the blob.
+                            float reg = 0.0f;
+#pragma unroll
+                            for (int i = 0; i < EK; ++i)
+                                reg = fmaf(ew[k + j][i], *(const float*)((const char*)lds + (eaddr[k + j][i] + hbase)),
+                                           reg);
+                            if ((xtra >> (k + j)) & 1u) {
+                                const uint32_t* __restrict__ ep = edges + 2 * (in_off[gg] + EK);
+#pragma unroll 1
+                                for (int n = in_off[gg + 1] - in_off[gg] - EK; n > 0; --n, ep += 2)
+                                    reg = fmaf(as_f(ep[1]), H[ep[0] * 64 + lane], reg);
+                            }
+                            // AND / OR gates (cle.jl:89-106); OR uses rep = 1 - act.
+                            if ((xtra >> (16 + k + j)) & 1u) {
+                                const int q1 = gate_off[gg + 1];
+                                for (int q = gate_off[gg]; q < q1;) {
+                                    const uint32_t hd = gates[q];
+                                    const uint32_t type = hd >> 24;
+                                    const int ns = (int)(hd & 0xffffu);
+                                    float prod = 1.0f;
+                                    for (int i = 0; i < ns; ++i)
+                                        prod *= H[(2 * (int)gates[q + 2 + i] + (int)type) * 64 + lane];
+                                    reg = fmaf(as_f(gates[q + 1]), type == GATE_AND ? prod : 1.0f - prod, reg);
+                                    q += 2 + ns;
+                                }
+                            }
+                            const float beta = beta_v[k + j];
+                            const bool o = __builtin_amdgcn_inverse_ballot_w64(on[k + j]);
+                            // telegraph gating of the transcription propensity
+                            // (tauleap_binomial.jl:134-137); volume scaling (cle.jl:224-225)
+                            const float drive = POP ? (reg + beta) * (float)V : reg + beta;
+                            const float tx = o ? drive : 0.0f;
+                            // mRNA (cle.jl:110-113)
+                            const float mm = m[k + j];
+                            const float var_m = fmaxf(fmaf(mu_m, fmaxf(mm, 0.0f), tx), 0.0f);
+                            float mn = fmaf(fmaf(-mu_m, mm, tx), dt, mm);
+                            mn = fmaxf(fmaf(c[j], __builtin_amdgcn_sqrtf(r2[j] * var_m), mn), 0.0f);
+                            // protein with the updated mRNA (cle.jl:116-120)
+                            const float pp = p[k + j];
+                            const float var_p = fmaxf(fmaf(k_t, mn, mu_p * fmaxf(pp, 0.0f)), 0.0f);
+                            float pn = fmaf(fmaf(k_t, mn, -mu_p * pp), dt, pp);
+                            pn = fmaxf(fmaf(sn[j], __builtin_amdgcn_sqrtf(r2[j] * var_p), pn), 0.0f);
+                            m[k + j] = mn;
+                            p[k + j] = pn;
+                        }
                     }
                 }
             }
         }
-        if (pop) V *= growth;                                 // cle.jl:237
+        if (POP && (fl & F_POP)) V *= __longlong_as_double((long long)u64_at(Bs, H_GROWTH_LO));   // cle.jl:237
         if (nbuf == 1) __syncthreads();
 
-        if (snap_every > 0 && ((s + 1) % snap_every) == 0 && live) {
-            const int snap = (s + 1) / snap_every - 1;
+        if (s == next_snap) {
+            const int se = (int)Bs[H_SNAP_EVERY];
+            next_snap += se;
+            if (live) {
+                const int snap = (s + 1) / se - 1;
+                const int Gs = (int)Bs[H_G];
+                const size_t Cs = (size_t)Bs[H_CELLS];
+                float* __restrict__ out = snap_pool + (size_t)u64_at(Bs, H_OUT_SNAP_LO);
 #pragma unroll
-            for (int k = 0; k < CH; ++k) {
-                const int g = g0 + k;
-                if (g < G) {
-                    snap_pool[off_snap + ((size_t)(2 * snap) * G + g) * C + cell] = m[k];
-                    snap_pool[off_snap + ((size_t)(2 * snap + 1) * G + g) * C + cell] = p[k];
+                for (int k = 0; k < CH; ++k) {
+                    if (k < nv) {
+                        out[((size_t)(2 * snap) * Gs + g0 + k) * Cs + cell] = m[k];
+                        out[((size_t)(2 * snap + 1) * Gs + g0 + k) * Cs + cell] = p[k];
+                    }
                 }
             }
         }
@@ -300,7 +431,7 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
             if (g < G) {
                 state_pool[off_state + (size_t)g * C + cell] = m[k];
                 state_pool[off_state + (size_t)(G + g) * C + cell] = p[k];
-                prom_pool[off_prom + (size_t)g * C + cell] = (uint8_t)((on >> k) & 1u);
+                prom_pool[off_prom + (size_t)g * C + cell] = (uint8_t)__builtin_amdgcn_inverse_ballot_w64(on[k]);
             }
         }
         if (pop && wave == 0) vol_pool[off_vol + cell] = V;
@@ -522,8 +653,27 @@ hipError_t launch_pop_place(int n_tiles, const uint32_t* blobs, const Tile* tile
     return hipGetLastError();
 }
 
-size_t stepper_lds_bytes(int S, int nbuf) { return (size_t)nbuf * 2 * (size_t)S * 64 * sizeof(float); }
+// Per buffer: 2S Hill rows (act, rep of each source slot) and one zero row.
+size_t stepper_lds_bytes(int S, int nbuf) { return (size_t)nbuf * (2 * (size_t)S + 1) * 64 * sizeof(float); }
 
+template <int CH>
+static const void* stepper_fn(bool inj, bool pop) {
+    if (inj) return pop ? (const void*)cle_stepper_kernel<CH, true, true> : (const void*)cle_stepper_kernel<CH, true, false>;
+    return pop ? (const void*)cle_stepper_kernel<CH, false, true> : (const void*)cle_stepper_kernel<CH, false, false>;
+}
+
+static const void* stepper_fn(int CH, bool inj, bool pop) {
+    switch (CH) {
+        case 2: return stepper_fn<2>(inj, pop);
+        case 4: return stepper_fn<4>(inj, pop);
+        case 8: return stepper_fn<8>(inj, pop);
+        case 16: return stepper_fn<16>(inj, pop);
+        default: return nullptr;
+    }
+}
+
+// A launch with no injected-noise pool (no network of it injects noise) or no
+// volume pool (no population) runs the kernel variant without that path.
 hipError_t launch_stepper(int CH, int NW, int n_tiles, const uint32_t* blobs, const Tile* tiles,
                           float* state_pool, float* snap_pool, uint8_t* prom_pool, double* vol_pool,
                           const float* inj_pool, int s_begin, int s_end, int nbuf,
@@ -531,23 +681,12 @@ hipError_t launch_stepper(int CH, int NW, int n_tiles, const uint32_t* blobs, co
     if (n_tiles <= 0) return hipSuccess;
     if (sched == nullptr) n_seg = 1;
     if (n_seg < 1 || (long long)n_tiles * n_seg > INT32_MAX) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)(n_tiles * n_seg)), block(64 * NW);
-#define SCRNA_STEPPER_CASE(N)                                                                                  \
-    case N:                                                                                                    \
-        hipLaunchKernelGGL(cle_stepper_kernel<N>, grid, block, lds_bytes, stream, blobs, tiles, state_pool,   \
-                           snap_pool, prom_pool, vol_pool, inj_pool, s_begin, s_end, nbuf, sched, n_tiles, n_seg, \
-                           wait_ticks);                                                                        \
-        break;
-    switch (CH) {
-        SCRNA_STEPPER_CASE(2)
-        SCRNA_STEPPER_CASE(4)
-        SCRNA_STEPPER_CASE(8)
-        SCRNA_STEPPER_CASE(16)
-        default:
-            return hipErrorInvalidValue;
-    }
-#undef SCRNA_STEPPER_CASE
-    return hipGetLastError();
+    const void* f = stepper_fn(CH, inj_pool != nullptr, vol_pool != nullptr);
+    if (!f) return hipErrorInvalidValue;
+    void* args[] = {(void*)&blobs, (void*)&tiles, (void*)&state_pool, (void*)&snap_pool, (void*)&prom_pool,
+                    (void*)&vol_pool, (void*)&inj_pool, (void*)&s_begin, (void*)&s_end, (void*)&nbuf,
+                    (void*)&sched, (void*)&n_tiles, (void*)&n_seg, (void*)&wait_ticks};
+    return hipLaunchKernel(f, dim3((unsigned)(n_tiles * n_seg)), dim3(64 * NW), args, lds_bytes, stream);
 }
 
 hipError_t launch_sched_reset(int* sched, int n_words, hipStream_t stream) {
@@ -558,14 +697,8 @@ hipError_t launch_sched_reset(int* sched, int n_words, hipStream_t stream) {
 }
 
 int stepper_blocks_per_cu(int CH, int NW, size_t lds_bytes) {
-    const void* f = nullptr;
-    switch (CH) {
-        case 2: f = (const void*)cle_stepper_kernel<2>; break;
-        case 4: f = (const void*)cle_stepper_kernel<4>; break;
-        case 8: f = (const void*)cle_stepper_kernel<8>; break;
-        case 16: f = (const void*)cle_stepper_kernel<16>; break;
-        default: return 0;
-    }
+    const void* f = stepper_fn(CH, false, false);
+    if (!f) return 0;
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 64 * NW, lds_bytes) != hipSuccess) return 0;
     return nb;
@@ -575,12 +708,12 @@ hipError_t set_stepper_lds_limit(size_t bytes) {
     // The kernel's own static LDS (schedule ticket and flag) shares the budget.
     bytes = bytes > kStepperStaticLds ? bytes - kStepperStaticLds : bytes;
     hipError_t e = hipSuccess;
-    const void* fns[] = {(const void*)cle_stepper_kernel<2>, (const void*)cle_stepper_kernel<4>,
-                         (const void*)cle_stepper_kernel<8>, (const void*)cle_stepper_kernel<16>};
-    for (const void* f : fns) {
-        hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-        if (r != hipSuccess) e = r;
-    }
+    for (int CH : {2, 4, 8, 16})
+        for (int v = 0; v < 4; ++v) {
+            hipError_t r = hipFuncSetAttribute(stepper_fn(CH, v & 1, v & 2), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)bytes);
+            if (r != hipSuccess) e = r;
+        }
     return e;
 }
 

@@ -29,14 +29,25 @@ struct U4 {
     uint32_t x, y, z, w;
 };
 
+// Three-input xor in one gfx950 VALU instruction (v_bitop3_b32, truth table
+// 0x96); the compiler does not form it from two xors by itself.
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
+// The kernels call it with c0 per lane and c1..c3 wave-uniform.  Then the
+// first rounds' xors are scalar (c1 ^ k0 and c3 ^ k1 are uniform until the
+// per-lane word has reached them), and a VALU bitop3 there would pull uniform
+// values into VGPRs; from round 2 (n0) and round 3 (n2) on every operand but
+// the key is per lane and one bitop3 replaces two v_xor_b32.
 __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                              uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c0;
         const uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
-        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        const uint32_t n0 = r >= 2 ? xor3((uint32_t)(p1 >> 32), c1, k0) : (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = r >= 3 ? xor3((uint32_t)(p0 >> 32), c3, k1) : (uint32_t)(p0 >> 32) ^ c3 ^ k1;
         c1 = (uint32_t)p1;
         c3 = (uint32_t)p0;
         c0 = n0;
@@ -58,17 +69,38 @@ __device__ __forceinline__ U4 philox4x32_10_uk(uint32_t c0, uint32_t c1, uint32_
     return philox4x32_10(c0, c1, c2, c3, k0, k1);
 }
 
-// Box-Muller on two u32.  The radius uniform ((a>>9)+1/2)*2^-23 and the
-// angle (b>>8)*2^-24 (revolutions) are exact in float, so the CPU oracle's
-// double transform sees the same uniforms.  neg2dt = -2*ln(2)*scale^2
-// folds the log2->ln conversion and a scale (sqrt(dt)) into the radius.
+// 1 + (w >> 9) * 2^-23 as a float in [1, 2): the top 23 bits of w placed
+// under the exponent of 1.0 by one v_alignbit_b32 ((0x7f:w) >> 9).
+__device__ __forceinline__ float one_plus_u23(uint32_t w) {
+    return __uint_as_float(__builtin_amdgcn_alignbit(0x7Fu, w, 9u));
+}
+
+// Box-Muller on two u32.  The radius uniform u1 = ((a>>9)+1/2)*2^-23 and the
+// angle (b>>9)*2^-23 (revolutions) are exact in float, so the CPU oracle's
+// double transform sees the same uniforms.  u1 is formed as
+// (1 + (a>>9)*2^-23) - (1 - 2^-24), which is exact (the difference has at most
+// 24 significant bits); the angle is passed as 1 + (b>>9)*2^-23, one whole
+// revolution more, which v_sin/v_cos (periodic in revolutions, valid on
+// [-256, 256]) do not see.  neg2ln2s2 = -2*ln(2)*scale^2 folds the log2->ln
+// conversion and a scale (sqrt(dt)) into the radius.
 __device__ __forceinline__ void normal_pair_scaled(uint32_t a, uint32_t b, float neg2ln2s2,
                                                    float& z0, float& z1) {
-    const float u1 = ((float)(a >> 9) + 0.5f) * 0x1p-23f;
+    const float u1 = one_plus_u23(a) - (1.0f - 0x1p-24f);
     const float r = __builtin_amdgcn_sqrtf(neg2ln2s2 * __builtin_amdgcn_logf(u1));
-    const float th = (float)(b >> 8) * 0x1p-24f;   // revolutions in [0,1)
+    const float th = one_plus_u23(b);
     z0 = r * __builtin_amdgcn_cosf(th);             // cos(2*pi*th)
     z1 = r * __builtin_amdgcn_sinf(th);             // sin(2*pi*th)
+}
+
+// The same pair as radius squared and direction: z0 = c*sqrt(r2), z1 = sn*sqrt(r2),
+// for callers that fold sqrt(r2) into another square root.
+__device__ __forceinline__ void normal_pair_r2(uint32_t a, uint32_t b, float neg2ln2s2, float& r2, float& c,
+                                               float& sn) {
+    const float u1 = one_plus_u23(a) - (1.0f - 0x1p-24f);
+    r2 = neg2ln2s2 * __builtin_amdgcn_logf(u1);
+    const float th = one_plus_u23(b);
+    c = __builtin_amdgcn_cosf(th);
+    sn = __builtin_amdgcn_sinf(th);
 }
 
 // 52-bit uniform in (0,1) from two u32 (capture and population draws).

@@ -21,12 +21,15 @@ struct SegmentSlot {
 };
 
 // Ticket -> (tile, segment); without a schedule the workgroup is its tile.
+// The ticket comes back through LDS; readfirstlane states that it is uniform,
+// so the tile descriptor and the blob behind it are read with scalar loads.
 __device__ __forceinline__ SegmentSlot segment_take(int* __restrict__ sched, int n_tiles, int* sh_ticket) {
     if (sched == nullptr) return SegmentSlot{(int)blockIdx.x, 0};
     if (threadIdx.x == 0) *sh_ticket = atomicAdd(&sched[SCHED_TICKET], 1);
     __syncthreads();
-    const int seg = *sh_ticket / n_tiles;
-    return SegmentSlot{*sh_ticket - seg * n_tiles, seg};
+    const int t = __builtin_amdgcn_readfirstlane(*sh_ticket);
+    const int seg = t / n_tiles;
+    return SegmentSlot{t - seg * n_tiles, seg};
 }
 
 // Waits for segment seg-1 of the tile; returns false (after releasing this

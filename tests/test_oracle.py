@@ -46,7 +46,7 @@ def test_philox_rocrand_kat():
 def test_normal_pair_identities(a, b):
     z0, z1 = orc.normal_pair(a, b)
     u1 = ((a >> 9) + 0.5) * 2.0 ** -23
-    th = (b >> 8) * 2.0 ** -24
+    th = (b >> 9) * 2.0 ** -23
     assert math.isclose(z0 * z0 + z1 * z1, -2.0 * math.log(u1), rel_tol=1e-13)
     if abs(z0) + abs(z1) > 0:
         ang = math.atan2(z1, z0) % (2 * math.pi)
