@@ -48,6 +48,21 @@ __device__ __forceinline__ uint32_t pick4(const U4& u, int i) {
 
 constexpr int kDenseMaxKC = 8;   // Gp <= 256
 
+// Cross-lane moves inside a quad of lanes as DPP operands of a v_mov (no LDS
+// round trip as with ds_bpermute).  quad_from<R>: lane j receives lane
+// (j - R) & 3's value; xor1: lane j receives lane j ^ 1's value.
+__device__ __forceinline__ uint32_t dpp_xor1(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+}
+__device__ __forceinline__ uint32_t quad_from(uint32_t x, int r) {
+    switch (r & 3) {
+        case 1: return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x93, 0xF, 0xF, false);   // [3,0,1,2]
+        case 2: return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, false);   // [2,3,0,1]
+        case 3: return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x39, 0xF, 0xF, false);   // [1,2,3,0]
+        default: return x;
+    }
+}
+
 template <int MODE, int NB, bool TELE, bool INJECT>
 __global__ void __launch_bounds__(1024 / NB)
 cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ tiles,
@@ -76,7 +91,6 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
     const float mu_p = d_as_f(B[H_MUP]);
     const float invK = d_as_f(B[H_INVK]);
     const float hill_n = d_as_f(B[H_HILLN]);
-    const float sqrt_dt = d_as_f(B[H_SQRTDT]);
     const bool hill2 = (flags & F_HILL2) != 0;
     const bool tele = TELE;                 // specialised: telegraph and injected-noise paths cost registers
     const bool inject = INJECT;
@@ -102,10 +116,17 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
     const uint32_t* __restrict__ gates = B + B[H_OFF_GATES];
     const uint32_t* __restrict__ telew = B + B[H_OFF_TELE];
 
-    // Per gene block: weights, constants, state.
+    // Per gene block: weights, constants, state.  A gene's gate, when it has
+    // exactly one and it has two sources (the form of every gate the prior
+    // draws), lives in registers: strength*beta, OR flag and the offsets of
+    // its sources' Hill rows (AND reads act rows, OR rep rows).  Other gate
+    // layouts take the loop over the blob's records (gslow).
     bf16x8 whi[NB][kDenseMaxKC], wlo[NB][kDenseMaxKC];
     int g[NB], slot[NB], q0[NB], q1[NB];
     bool gl[NB];
+    float gw[NB];
+    int ga[NB], gb[NB];
+    bool gor[NB], gslow[NB], wave_gate[NB], wave_slow[NB];
     float beta[NB], bias[NB];
     uint32_t tflags[NB];
     uint64_t thr_on[NB], thr_off[NB];
@@ -137,6 +158,25 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
         thr_off[b] = d_u64(telew, gc * 8 + T_OFF_LO);
         q0[b] = gl[b] ? gate_off[g[b]] : 0;
         q1[b] = gl[b] ? gate_off[g[b] + 1] : 0;
+        gw[b] = 0.0f;
+        ga[b] = gb[b] = 0;
+        gor[b] = gslow[b] = false;
+        bool greg = false;
+        if (q1[b] > q0[b]) {
+            const uint32_t hd = gates[q0[b]];
+            const int type = (int)(hd >> 24), ns = (int)(hd & 0xffffu);
+            if (ns == 2 && q1[b] - q0[b] == 2 + ns) {
+                greg = true;
+                gw[b] = d_as_f(gates[q0[b] + 1]);
+                gor[b] = type == (int)GATE_OR;
+                ga[b] = (2 * (int)gates[q0[b] + 2] + type) * 16;
+                gb[b] = (2 * (int)gates[q0[b] + 3] + type) * 16;
+            } else {
+                gslow[b] = true;
+            }
+        }
+        wave_gate[b] = __builtin_amdgcn_ballot_w64(greg) != 0;
+        wave_slow[b] = __builtin_amdgcn_ballot_w64(gslow[b]) != 0;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             m[b][v] = 0.0f;
@@ -163,6 +203,8 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
     }
 
     const int s_stop = min(min(s_end, tile.s1), n_steps);
+    // Next step after which a snapshot is due (a countdown, not a division per step).
+    int next_snap = snap_every > 0 ? (s_start / snap_every + 1) * snap_every - 1 : INT32_MAX;
     for (int s = s_start; s < s_stop; ++s) {
         const int buf = s & 1;
         __bf16* T0 = act_t + (size_t)buf * tile_elems;
@@ -179,10 +221,9 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
                 const U4 u = philox4x32_10_uk((uint32_t)cell(l & 3), (uint32_t)s, (uint32_t)(g[b] >> 2), c3s, k0, k1);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int j = l & 3;
-                    const int src = (l & ~3) | ((j - r) & 3);      // drew element (j - r) & 3
-                    const uint32_t send = pick4(u, (j + r) & 3);    // word of receiver (j + r) & 3
-                    const uint32_t got = (uint32_t)__shfl((int)send, src);
+                    const int j = l & 3;                            // receives from lane (j - r) & 3,
+                    const uint32_t send = pick4(u, (j + r) & 3);    // which drew element (j - r) & 3
+                    const uint32_t got = quad_from(send, r);
                     const int bit = 4 * b + ((j - r) & 3);
                     if (tflags[b] & 1u) {
                         const bool cur = (on >> bit) & 1u;
@@ -198,14 +239,12 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
                 constexpr bool H2 = decltype(h2)::value;
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    float act = 0.0f, rep = 1.0f;
-                    if (gl[b]) {
-                        const float x = p[b][v] * invK;
-                        float xn = H2 ? x * x : __builtin_amdgcn_exp2f(hill_n * __builtin_amdgcn_logf(x));
-                        xn = fminf(xn, 1e30f);
-                        rep = __builtin_amdgcn_rcpf(1.0f + xn);
-                        act = xn * rep;
-                    }
+                    // Padding genes (g >= G) keep p = 0, so act = 0 and rep = 1.
+                    const float x = p[b][v] * invK;
+                    float xn = H2 ? x * x : __builtin_amdgcn_exp2f(hill_n * __builtin_amdgcn_logf(x));
+                    xn = fminf(xn, 1e30f);
+                    const float rep = __builtin_amdgcn_rcpf(1.0f + xn);
+                    const float act = xn * rep;
                     const __bf16 hi = (__bf16)act;
                     T0[row(v) * rowstride + g[b]] = hi;
                     if (MODE == 2) T1[row(v) * rowstride + g[b]] = (__bf16)(act - (float)hi);
@@ -246,27 +285,37 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
         }
 
         // Elementwise update of element (b, v) (cle.jl:108-120).
-        auto update = [&](const int b, const int v, float xm, float xp) {
+        // Noise enters as xi_m = c*sqrt(r2), xi_p = sn*sqrt(r2) (Box-Muller radius
+        // squared, scaled by dt), so xi*sqrt(var) costs one square root, sqrt(r2*var).
+        auto update = [&](const int b, const int v, float r2, float c, float sn) {
             float reg = acc[b][v] + bias[b];
-            for (int q = q0[b]; q < q1[b];) {                   // AND / OR (cle.jl:89-106)
-                const uint32_t hd = gates[q];
-                const uint32_t type = hd >> 24;
-                const int ns = (int)(hd & 0xffffu);
-                float prod = 1.0f;
-                for (int i = 0; i < ns; ++i) prod *= GH[(2 * (int)gates[q + 2 + i] + (int)type) * 16 + row(v)];
-                reg = fmaf(d_as_f(gates[q + 1]), type == GATE_AND ? prod : 1.0f - prod, reg);
-                q += 2 + ns;
+            // AND / OR (cle.jl:89-106).  Lanes without a register gate read a
+            // valid Hill row with weight 0.
+            if (wave_gate[b]) {
+                const float prod = GH[ga[b] + row(v)] * GH[gb[b] + row(v)];
+                reg = fmaf(gw[b], gor[b] ? 1.0f - prod : prod, reg);
+            }
+            if (wave_slow[b] && gslow[b]) {
+                for (int q = q0[b]; q < q1[b];) {
+                    const uint32_t hd = gates[q];
+                    const uint32_t type = hd >> 24;
+                    const int ns = (int)(hd & 0xffffu);
+                    float prod = 1.0f;
+                    for (int i = 0; i < ns; ++i) prod *= GH[(2 * (int)gates[q + 2 + i] + (int)type) * 16 + row(v)];
+                    reg = fmaf(d_as_f(gates[q + 1]), type == GATE_AND ? prod : 1.0f - prod, reg);
+                    q += 2 + ns;
+                }
             }
             const bool o = (on >> (4 * b + v)) & 1u;
             const float tx = o ? reg + beta[b] : 0.0f;
             const float mm = m[b][v];
             const float var_m = fmaxf(fmaf(mu_m, fmaxf(mm, 0.0f), tx), 0.0f);
             float mn = fmaf(fmaf(-mu_m, mm, tx), dt, mm);
-            mn = fmaxf(fmaf(xm, __builtin_amdgcn_sqrtf(var_m), mn), 0.0f);
+            mn = fmaxf(fmaf(c, __builtin_amdgcn_sqrtf(r2 * var_m), mn), 0.0f);
             const float pp = p[b][v];
             const float var_p = fmaxf(fmaf(k_t, mn, mu_p * fmaxf(pp, 0.0f)), 0.0f);
             float pn = fmaf(fmaf(k_t, mn, -mu_p * pp), dt, pp);
-            pn = fmaxf(fmaf(xp, __builtin_amdgcn_sqrtf(var_p), pn), 0.0f);
+            pn = fmaxf(fmaf(sn, __builtin_amdgcn_sqrtf(r2 * var_p), pn), 0.0f);
             m[b][v] = gl[b] ? mn : 0.0f;
             p[b][v] = gl[b] ? pn : 0.0f;
         };
@@ -277,8 +326,7 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
                 const size_t base = ((size_t)s * G + (gl[b] ? g[b] : 0)) * 2;
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
-                    update(b, v, inj_pool[off_inj + base * C + cell(v)] * sqrt_dt,
-                           inj_pool[off_inj + (base + 1) * C + cell(v)] * sqrt_dt);
+                    update(b, v, dt, inj_pool[off_inj + base * C + cell(v)], inj_pool[off_inj + (base + 1) * C + cell(v)]);
             } else {
                 // The normal block (cell, s, g>>1) carries genes g (words 0,1) and
                 // g^1 (words 2,3), i.e. lanes l and l^1, which own the same cells.
@@ -289,18 +337,19 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
                     const int cm = par ? cell(t + 2) : cell(t);
                     const U4 u = philox4x32_10_uk((uint32_t)cm, (uint32_t)s, (uint32_t)(g[b] >> 1), c3n, k0, k1);
                     const uint32_t mine_a = par ? u.z : u.x, mine_b = par ? u.w : u.y;
-                    const uint32_t got_a = (uint32_t)__shfl_xor((int)(par ? u.x : u.z), 1);
-                    const uint32_t got_b = (uint32_t)__shfl_xor((int)(par ? u.y : u.w), 1);
-                    float xm, xp;
-                    normal_pair_scaled(par ? got_a : mine_a, par ? got_b : mine_b, neg2ln2dt, xm, xp);
-                    update(b, t, xm, xp);
-                    normal_pair_scaled(par ? mine_a : got_a, par ? mine_b : got_b, neg2ln2dt, xm, xp);
-                    update(b, t + 2, xm, xp);
+                    const uint32_t got_a = dpp_xor1(par ? u.x : u.z);
+                    const uint32_t got_b = dpp_xor1(par ? u.y : u.w);
+                    float r2, c, sn;
+                    normal_pair_r2(par ? got_a : mine_a, par ? got_b : mine_b, neg2ln2dt, r2, c, sn);
+                    update(b, t, r2, c, sn);
+                    normal_pair_r2(par ? mine_a : got_a, par ? mine_b : got_b, neg2ln2dt, r2, c, sn);
+                    update(b, t + 2, r2, c, sn);
                 }
             }
         }
 
-        if (snap_every > 0 && ((s + 1) % snap_every) == 0) {
+        if (s == next_snap) {
+            next_snap += snap_every;
             const int snap = (s + 1) / snap_every - 1;
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
