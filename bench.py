@@ -24,6 +24,7 @@ import ctypes
 import json
 import os
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -312,6 +313,12 @@ def run_batch(a, sc) -> dict:
     nets, cells, pop = samplers.batch_networks(n)
     kin = samplers.WORKLOAD_KINETICS
     T, dt = a.T or 100.0, 0.01
+    # Tile shapes (genes per lane per G bucket) from the autotuner, timed on this device.
+    t0 = time.perf_counter()
+    tune_path = os.path.join(tempfile.mkdtemp(), "tiles.json")
+    sc.default_context().autotune(max_G=max(x.G for x in nets), cache_path=tune_path)
+    t_tune = time.perf_counter() - t0
+    tile_table = json.load(open(tune_path))["table"]
     pops = [sc.PopulationConfig(cell_num=c, growth_rate=pop.growth_rate, V_div=pop.V_div, V_init=pop.V_init,
                                 div_check_interval=pop.div_check_interval, partition=pop.partition,
                                 replace=pop.replace) for c in cells]
@@ -330,7 +337,7 @@ def run_batch(a, sc) -> dict:
             "networks": n, "total_cells": int(sum(cells)), "mean_G": float(np.mean([x.G for x in nets])),
             "cell_gene_steps": cgs, "cgs_per_s": cgs * a.steps / el, "s_per_run": el / a.steps,
             "graph_kernel_ms": float(np.mean(kms)), "plan_build_s": t_build, "graph_capture_s": t_graph,
-            "finite": bool(np.all(np.isfinite(sample.y)))}
+            "autotune_s": t_tune, "tile_table": tile_table, "finite": bool(np.all(np.isfinite(sample.y)))}
 
 
 def run_serve(a, sc) -> dict:

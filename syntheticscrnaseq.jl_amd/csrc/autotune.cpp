@@ -173,6 +173,10 @@ int scrna_autotune(scrna_ctx* ctx, int32_t max_G, const char* cache_path, int32_
         return SCRNA_OK;
     }
     static const int buckets[] = {4, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256};
+    // Enough cells for eight 64-cell tiles per CU, so that every shape is timed
+    // at the occupancy it reaches in a full launch rather than on a half-empty
+    // device (where the shape with the most waves per tile always wins).
+    const int64_t cells = std::max<int64_t>(16384, (int64_t)ctx->num_cu * 64 * 8);
     std::vector<TileRule> rules;
     int prev = 0;
     for (int Gb : buckets) {
@@ -184,7 +188,7 @@ int scrna_autotune(scrna_ctx* ctx, int32_t max_G, const char* cache_path, int32_
         for (int ch : {2, 4, 8, 16}) {
             if ((Gb + ch - 1) / ch > 16) continue;
             double ms = 0;
-            int rc = time_shape(ctx, s, ch, 16384, 64, &ms);
+            int rc = time_shape(ctx, s, ch, cells, 64, &ms);
             if (rc) return rc;
             if (ms < best) { best = ms; best_ch = ch; }
         }

@@ -678,7 +678,11 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
             lg.lds = tl ? tauleap_lds_bytes(S, lg.nbuf, lg.NW, lg.CH) : stepper_lds_bytes(S, lg.nbuf);
             const int per_cu = tl ? tauleap_blocks_per_cu(lg.NW, lg.lds)
                                   : stepper_blocks_per_cu(lg.CH, lg.NW, lg.lds);
-            lg.nseg = choose_segments(lg.n_tiles, (long long)per_cu * ctx->num_cu, group_steps[kv.first], tl ? 8 : 16);
+            // Population plans advance in division phases that always run one
+            // workgroup per tile (launch_steps with segmented = false).
+            lg.nseg = P->has_population ? 1
+                                        : choose_segments(lg.n_tiles, (long long)per_cu * ctx->num_cu,
+                                                          group_steps[kv.first], tl ? 8 : 16);
             if (lg.nseg > 1) {
                 lg.sched_off = P->sched_words;
                 P->sched_words += sched_words(lg.n_tiles);
@@ -775,6 +779,12 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
     PLAN_ALLOC(d_wpool, uint16_t, w_elems);
     PLAN_ALLOC(d_sched, int, P->sched_words);
 #undef PLAN_ALLOC
+    // Every launch clears its block with a kernel first; clearing it here as
+    // well means no error word is ever read uninitialised.
+    if (P->d_sched && hipMemset(P->d_sched, 0, P->sched_words * sizeof(int)) != hipSuccess) {
+        plan_release(P.get());
+        return set_error(SCRNA_ERR_DEVICE, "hipMemset of the schedule block failed");
+    }
     P->wait_ticks = 300000LL * ctx->wall_khz;   // 300 s
     P->n_cjobs = (int)cjobs.size();
     for (auto& u : P->unfused) {

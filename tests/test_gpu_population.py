@@ -74,3 +74,24 @@ def test_population_reference_smoke(ctx):
                               partition="halve", replace="uniform")
     y = sc.simulate(net, sc.CLE(0.1), kin, population=pop, T=100.0, readout="protein", seed=1)
     assert y.shape == (500, 1) and np.all(np.isfinite(y)) and y.mean() > 0
+
+
+def test_large_population_plan_syncs_after_dirty_memory(ctx):
+    """A population plan with more tiles than the device holds at once never
+    runs the segmented schedule, so its completion check must not read a
+    schedule block (one that no launch cleared held stale values from earlier
+    allocations and was reported as a timed-out segment)."""
+    net = sc.GeneNetwork(2, np.array([2.0, 1.0]), np.array([[0, 1.5], [0, 0]]))
+    big = sc.Plan([net], sc.CLE(0.01), KIN, cell_num=300_000, T=0.64, readout="both", seed=1)
+    big.launch()
+    big.sync()
+    big.close()                                      # frees memory holding non-zero state
+    pop = sc.PopulationConfig(cell_num=200_000, growth_rate=0.1, V_div=2.0, V_init=(0.8, 1.2),
+                              div_check_interval=4)
+    plan = sc.Plan([net], sc.CLE(0.01), KIN, population=pop, T=0.2, readout="both", seed=2)
+    for _ in range(2):
+        plan.launch()
+        plan.sync()
+    out = plan.download(0)
+    assert out.y.shape == (200_000, 4) and np.all(np.isfinite(out.y))
+    plan.close()
