@@ -236,9 +236,13 @@ int compile_network(const scrna_network* net, const scrna_kinetics* kin, const s
         if (NW > 16) return set_error(SCRNA_ERR_INVALID, "tile override CH=%d gives %d waves > 16", CH, NW);
     }
     if (run->algorithm == SCRNA_ALG_BINOMIAL_TAU_LEAP) {
-        // Tau-leap: genes spread over at most 8 waves (the FP64 samplers need
-        // the register budget of 2 waves per SIMD); genes per wave is a run-time value.
-        NW = std::min(tauleap_max_waves(), G);
+        // Tau-leap: the FP64 samplers hold the kernel at 2 waves per SIMD, 8 per
+        // CU.  Workgroups of 4 waves let two share a CU, so one computes while
+        // the other waits at its per-step barrier behind its slowest wave (the
+        // samplers' loop counts vary by lane); more waves only where a wave would
+        // exceed 32 genes.  G=20 benchmark: 233 ms at 4 waves, 237 at 2, 321 at 7.
+        // Genes per wave is a run-time value.
+        NW = std::min(tauleap_max_waves(), std::max(std::min(4, G), (G + 31) / 32));
         CH = (G + NW - 1) / NW;
         NW = (G + CH - 1) / CH;
     }
