@@ -150,8 +150,11 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
                 if (MODE == 2) wlo[b][kc] = wpool[off_w + ((size_t)NBLK * KC + (size_t)blk * KC + kc) * 64 + l];
             }
         }
-        beta[b] = d_as_f(B[B[H_OFF_BETA] + gc]);
-        bias[b] = d_as_f(B[B[H_OFF_BIAS] + gc]);
+        // Padding genes (g >= G) get beta = bias = 0, zero weight rows and
+        // columns, an OFF promoter and zero state, so every step leaves them at
+        // exactly 0 and the update needs no mask.
+        beta[b] = gl[b] ? d_as_f(B[B[H_OFF_BETA] + gc]) : 0.0f;
+        bias[b] = gl[b] ? d_as_f(B[B[H_OFF_BIAS] + gc]) : 0.0f;
         slot[b] = gl[b] ? ((const int32_t*)(B + B[H_OFF_SLOT]))[g[b]] : -1;
         tflags[b] = gl[b] ? telew[g[b] * 8 + T_FLAGS] : 0u;
         thr_on[b] = d_u64(telew, gc * 8 + T_ON_LO);
@@ -316,8 +319,8 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
             const float var_p = fmaxf(fmaf(k_t, mn, mu_p * fmaxf(pp, 0.0f)), 0.0f);
             float pn = fmaf(fmaf(k_t, mn, -mu_p * pp), dt, pp);
             pn = fmaxf(fmaf(sn, __builtin_amdgcn_sqrtf(r2 * var_p), pn), 0.0f);
-            m[b][v] = gl[b] ? mn : 0.0f;
-            p[b][v] = gl[b] ? pn : 0.0f;
+            m[b][v] = mn;
+            p[b][v] = pn;
         };
 
 #pragma unroll
