@@ -419,8 +419,8 @@ def main() -> int:
     ap.add_argument("--bytes", type=int, default=1 << 30, help="copy: bytes per copy")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(cells and networks would shard across GPUs with no exchange; see DESIGN.md).",
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
+              "(cells and networks would shard across GPUs with no exchange during a run).",
               file=sys.stderr)
         return 2
     if a.workload == "copy":
