@@ -83,6 +83,12 @@ enum TlWord : int {
     TL_HEAD = 16
 };
 
+// After the initial counts the tau-leap FP64 section holds three tables over
+// a count n < kTlTable: exp(-(k_t n dt)) (translation's Poisson start),
+// exp(n log q_m) and exp(n log q_p) (the decay binomials' inversion start,
+// q the complement of the decay probability or, above 1/2, the probability).
+constexpr int kTlTable = 256;
+
 enum BlobFlags : uint32_t {
     F_TELE = 1u,      // some gene has a finite switching rate
     F_HILL2 = 2u,     // Hill coefficient is exactly 2

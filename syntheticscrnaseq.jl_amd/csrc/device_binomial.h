@@ -56,8 +56,10 @@ __device__ __forceinline__ double stirling_fc(int64_t k) {
 
 // lq: nullptr, or {log(1-p), log(p)} precomputed by the caller, in which case
 // the inversion start q^n is exp(n * log q) instead of pow(q, n) (a fixed
-// decay probability drawn every step).
-__device__ inline int64_t binomial_sample(int64_t n, double p, UStream& st, const double* lq = nullptr) {
+// decay probability drawn every step).  r0_known >= 0 is that start value
+// itself, exp(n * log q) computed ahead (a table over n).
+__device__ inline int64_t binomial_sample(int64_t n, double p, UStream& st, const double* lq = nullptr,
+                                          double r0_known = -1.0) {
 #pragma clang fp contract(off)
     if (n <= 0 || !(p > 0.0)) return 0;
     if (p >= 1.0) return n;
@@ -67,7 +69,9 @@ __device__ inline int64_t binomial_sample(int64_t n, double p, UStream& st, cons
     int64_t x = 0;
     if ((double)n * pp < 10.0) {
         const double s = pp / q, a = (double)(n + 1) * s;
-        const double r0 = lq ? exp((double)n * (flip ? lq[1] : lq[0])) : pow(q, (double)n);
+        const double r0 = r0_known >= 0.0 ? r0_known
+                          : lq            ? exp((double)n * (flip ? lq[1] : lq[0]))
+                                          : pow(q, (double)n);
         for (;;) {
             double u = st.next(), r = r0;
             int64_t k = 0;
@@ -150,12 +154,12 @@ __device__ __forceinline__ double log_factorial(int64_t k) {
 // Exact Poisson(lam): sequential inversion from 0 for lam < 10, otherwise
 // PTRS (W. Hormann, "The transformed rejection method for generating Poisson
 // random variables", Insurance: Mathematics and Economics 12, 1993).  lam <= 0
-// draws nothing and returns 0.
-__device__ inline int64_t poisson_sample(double lam, UStream& st) {
+// draws nothing and returns 0.  p0_known >= 0 is exp(-lam) computed ahead.
+__device__ inline int64_t poisson_sample(double lam, UStream& st, double p0_known = -1.0) {
 #pragma clang fp contract(off)
     if (!(lam > 0.0)) return 0;
     if (lam < 10.0) {
-        const double p0 = exp(-lam);
+        const double p0 = p0_known >= 0.0 ? p0_known : exp(-lam);
         for (;;) {
             double u = st.next(), p = p0;
             int64_t k = 0;

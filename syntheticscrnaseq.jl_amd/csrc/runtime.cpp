@@ -478,6 +478,13 @@ int compile_network(const scrna_network* net, const scrna_kinetics* kin, const s
             d.push_back(run->init_mode == 0 ? std::nearbyint(mss) : 0.0);      // round half to even
             d.push_back(run->init_mode == 0 ? std::nearbyint(pss) : 0.0);
         }
+        // Start values of the inversion samplers over a count n, with the
+        // kernel's and the CPU restatement's expressions, so that the kernel
+        // reads the values the host library's exp returns.
+        for (int n = 0; n < kTlTable; ++n) d.push_back(std::exp(-(kin->k_t * (double)n * run->dt)));
+        for (int n = 0; n < kTlTable; ++n) d.push_back(std::exp((double)n * (pdm > 0.5 ? d[TL_LPM] : d[TL_LQM])));
+        for (int n = 0; n < kTlTable; ++n) d.push_back(std::exp((double)n * (pdp > 0.5 ? d[TL_LPP] : d[TL_LQP])));
+
         b[H_TL_NEDGES] = (uint32_t)out->n_edges;
         b[H_TL_NGATEW] = (uint32_t)gate_w.size();
         b[H_OFF_TL] = (uint32_t)b.size();   // 16-word aligned, so the doubles are 8-byte aligned

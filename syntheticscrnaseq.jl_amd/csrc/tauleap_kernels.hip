@@ -39,22 +39,29 @@ __device__ __forceinline__ uint64_t tl_u64(const uint32_t* B, int i) {
 
 // The four draws of one gene in one step (tauleap_binomial.jl:140-163), from
 // the gene's own uniform stream.
+// tab: the three start-value tables of the network (in LDS), each kTlTable
+// long: exp(-k_t n dt), exp(n log q_m), exp(n log q_p); counts beyond them
+// evaluate exp.
 __device__ __forceinline__ void tl_gene_step(int& m, int& p, double lam_tx, double k_t, double dt, double pdm,
-                                             const double* lqm, double pdp, const double* lqp, uint32_t cell,
-                                             uint32_t step, uint32_t g, uint32_t c3t, uint32_t k0, uint32_t k1) {
+                                             const double* lqm, double pdp, const double* lqp, const double* tab,
+                                             uint32_t cell, uint32_t step, uint32_t g, uint32_t c3t, uint32_t k0,
+                                             uint32_t k1) {
 #pragma clang fp contract(off)
     UStream st(cell, step, g << 16, c3t, k0, k1);
     const int n_tx = (int)poisson_sample(lam_tx, st);                                // :140
-    const int n_md = m > 0 ? (int)binomial_sample(m, pdm, st, lqm) : 0;             // :143-149
+    const int n_md = m > 0 ? (int)binomial_sample(m, pdm, st, lqm, m < kTlTable ? tab[kTlTable + m] : -1.0)
+                           : 0;                                                      // :143-149
     m += n_tx - n_md;                                                                // :150
     const int mpos = m > 0 ? m : 0;
-    const int n_tl = (int)poisson_sample(k_t * (double)mpos * dt, st);              // :153
-    const int n_pd = p > 0 ? (int)binomial_sample(p, pdp, st, lqp) : 0;             // :156-161
+    const int n_tl = (int)poisson_sample(k_t * (double)mpos * dt, st,
+                                         mpos < kTlTable ? tab[mpos] : -1.0);       // :153
+    const int n_pd = p > 0 ? (int)binomial_sample(p, pdp, st, lqp, p < kTlTable ? tab[2 * kTlTable + p] : -1.0)
+                           : 0;                                                      // :156-161
     p += n_tl - n_pd;                                                                // :163
 }
 
-// Dynamic LDS: Hill rows [nbuf][2S][64] doubles, then the lanes' counts
-// [NW][CH][2][64] int32.  Keeping the counts in LDS lets the gene loop stay
+// Dynamic LDS: Hill rows [nbuf][2S][64] doubles, then the sampler start-value
+// tables [3][kTlTable] doubles, then the lanes' counts [NW][CH][2][64] int32.  Keeping the counts in LDS lets the gene loop stay
 // rolled, so the exact samplers are inlined once rather than once per gene,
 // and CH (genes per wave, <= 32) is a run-time value.  At most 8 waves per
 // workgroup leave the FP64 samplers 256 VGPRs.
@@ -119,7 +126,12 @@ tauleap_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restric
     const int g0 = __builtin_amdgcn_readfirstlane(wave * CH);
     const int ng = __builtin_amdgcn_readfirstlane(max(0, min(CH, G - g0)));   // genes of this wave
     const int rows = 2 * S;
-    int* __restrict__ cnt = (int*)(tl_lds + (size_t)nbuf * rows * 64) + (size_t)wave * CH * 128;
+    double* __restrict__ tab = tl_lds + (size_t)nbuf * rows * 64;
+    int* __restrict__ cnt = (int*)(tab + 3 * kTlTable) + (size_t)wave * CH * 128;
+    {   // start-value tables -> LDS (read after the first step's barrier)
+        const double* __restrict__ src = init_d + 2 * G;
+        for (int i = threadIdx.x; i < 3 * kTlTable; i += blockDim.x) tab[i] = src[i];
+    }
     auto M = [&](int k) -> int& { return cnt[k * 128 + lane]; };
     auto P = [&](int k) -> int& { return cnt[k * 128 + 64 + lane]; };
 
@@ -223,8 +235,8 @@ tauleap_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restric
             if (pop) prop = prop * V;                                            // :221
             if (tele && !((on >> k) & 1u)) prop = 0.0;                           // :134-137
             int mk = M(k), pk = P(k);
-            tl_gene_step(mk, pk, prop * dt, k_t, dt, pdm, lqm, pdp, lqp, (uint32_t)cell, (uint32_t)s, (uint32_t)g,
-                         c3t, k0, k1);
+            tl_gene_step(mk, pk, prop * dt, k_t, dt, pdm, lqm, pdp, lqp, tab, (uint32_t)cell, (uint32_t)s,
+                         (uint32_t)g, c3t, k0, k1);
             M(k) = mk;
             P(k) = pk;
         }
@@ -255,7 +267,8 @@ tauleap_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restric
 }
 
 size_t tauleap_lds_bytes(int S, int nbuf, int NW, int CH) {
-    return (size_t)nbuf * 2 * (size_t)S * 64 * sizeof(double) + (size_t)NW * CH * 2 * 64 * sizeof(int);
+    return (size_t)nbuf * 2 * (size_t)S * 64 * sizeof(double) + (size_t)3 * kTlTable * sizeof(double) +
+           (size_t)NW * CH * 2 * 64 * sizeof(int);
 }
 
 hipError_t set_tauleap_lds_limit(size_t bytes) {
