@@ -90,9 +90,10 @@ constexpr int kWavesPerSimd4 = 7;
 // Scalar-register economy.  The step loop is long and every uniform value the
 // compiler keeps live across it takes an SGPR (a uniform boolean takes two, as
 // a lane mask); past the budget they spill into VGPR lanes and each reload is
-// a VALU v_readlane.  So the network flags and the switch mask are re-read as
-// opaque values at the top of every step: their per-gene tests are then
-// recomputed on the scalar unit instead of being hoisted as lane masks.
+// a VALU v_readlane.  So the blob offset and the per-wave bit masks are made
+// opaque at the top of every step: header words are re-read through the
+// scalar cache and per-gene tests recomputed on the scalar unit instead of
+// being hoisted (booleans as lane masks) out of the loop.
 template <int CH, bool INJ, bool POP>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(CH <= 4 ? kWavesPerSimd4 : 1)))
 cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ tiles,
