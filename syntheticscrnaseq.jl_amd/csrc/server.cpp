@@ -3,12 +3,18 @@
 // Requests are compiled on submit (host), admitted into a device arena when
 // it has room (the HBM budget), and advanced together: every round builds one
 // tile table covering all admitted requests, each tile carrying its request's
-// next chunk of steps, and launches the stepper once per kernel shape.
-// Requests that arrive while others are running join at the next round;
-// finished requests get their moment and capture kernels, are copied out and
-// release their arena ranges.  Results do not depend on the mix: random
-// streams are keyed by (seed, net_id, cell, step), and a chunk boundary only
-// stores and reloads the FP32 state.
+// next steps, and launches the stepper once per kernel shape.  Launches of
+// different shapes go to separate streams so that they run side by side;
+// a launch whose tiles fill the device runs the segmented schedule, so its
+// last wave of workgroups is short.  A request whose per-step cost is a small
+// fraction of the costliest active request's advances several chunks per
+// round (in a launch of its own), so small requests are not held to one
+// chunk per round at the pace of the largest.  Requests that arrive while
+// others are running join at the next round; finished requests get their
+// moment and capture kernels, are copied out and release their arena ranges.
+// Results do not depend on the mix: random streams are keyed by (seed,
+// net_id, cell, step), and a chunk or segment boundary only stores and
+// reloads the FP32 state and the promoter bits.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -18,6 +24,7 @@
 #include <memory>
 #include <mutex>
 #include <thread>
+#include <tuple>
 #include <unordered_map>
 #include <vector>
 
@@ -85,8 +92,9 @@ struct Req {
     // arena placement (bytes) and blob placement (words)
     size_t off_state = 0, off_prom = 0, off_counts = 0, off_mom = 0, off_blob = 0;
     size_t b_state = 0, b_prom = 0, b_counts = 0, b_mom = 0, w_blob = 0;
+    double cost = 0.0;                     // per-step cost estimate for the round planner
     Clock::time_point t_submit, t_done;
-    std::vector<double> y;
+    std::vector<float> y;                  // readout rows, [col][cell], converted to double on wait
     std::vector<int32_t> counts;
     std::vector<double> moments;
 };
@@ -96,7 +104,7 @@ struct Req {
 struct scrna_server {
     scrna_ctx* ctx = nullptr;
     int max_slots = 64;
-    int chunk = 256;                       // steps per round
+    int chunk = 256;                       // steps per round of the costliest requests
     void* arena = nullptr;
     size_t arena_bytes = 0;
     uint32_t* blob_arena = nullptr;
@@ -107,7 +115,13 @@ struct scrna_server {
     MomentJob* d_mjobs = nullptr;
     size_t jobs_cap = 0;
     std::unique_ptr<RangeAlloc> arena_alloc, blob_alloc;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;          // uploads, the first launch group and finishing work
+    static constexpr int kStreams = 4;     // launch groups run side by side on these (default priority)
+    hipStream_t side[kStreams - 1] = {};
+    hipEvent_t ev_tiles = nullptr;         // tile table uploaded, on `stream`
+    int* d_sched = nullptr;                // segmented-schedule blocks of one round's launches
+    size_t sched_cap = 0;
+    long long wait_ticks = 0;
 
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -165,6 +179,14 @@ bool admit(scrna_server* S, Req& r) {
     r.off_counts = o3; r.b_counts = b3;
     r.off_mom = o4; r.b_mom = b4;
     r.off_blob = ob / 4; r.w_blob = wb / 4;
+    // Per-step cost for the round planner: G x cells, with the cells floored
+    // at what fills the device's resident workgroup slots once (a request
+    // with fewer tiles takes about as long per step as one that fills them).
+    const int Smax = std::max(1, r.cn.S);
+    const int nbuf = stepper_lds_bytes(Smax, 2) <= S->ctx->lds_limit_soft ? 2 : 1;
+    const int per_cu = std::max(1, stepper_blocks_per_cu(r.cn.CH, r.cn.NW, stepper_lds_bytes(Smax, nbuf)));
+    const double fill = 64.0 * per_cu * std::max(1, S->ctx->num_cu);
+    r.cost = (double)r.cn.G * std::max((double)r.cn.cells, fill);
     std::vector<uint32_t>& b = r.cn.blob;
     auto put = [&](int idx, uint64_t v) { b[idx] = (uint32_t)v; b[idx + 1] = (uint32_t)(v >> 32); };
     put(H_OUT_FINAL_LO, o1 / sizeof(float));
@@ -245,12 +267,12 @@ void finish(scrna_server* S, std::vector<Req*>& done) {
     for (Req* r : done) {
         const size_t C = (size_t)r->cn.cells, G = (size_t)r->cn.G, GC = G * C;
         if (e == hipSuccess) {
-            std::vector<float> h(2 * GC);
-            e = hipMemcpy(h.data(), (char*)S->arena + r->off_state, 2 * GC * sizeof(float), hipMemcpyDeviceToHost);
+            // Only the rows the readout asks for; wait() converts them to double.
             const size_t ncol = r->readout == SCRNA_READOUT_BOTH ? 2 * G : G;
-            r->y.resize(ncol * C);
             const size_t row0 = r->readout == SCRNA_READOUT_PROTEIN ? G : 0;
-            for (size_t k = 0; k < ncol * C; ++k) r->y[k] = (double)h[row0 * C + k];
+            r->y.resize(ncol * C);
+            e = hipMemcpy(r->y.data(), (char*)S->arena + r->off_state + row0 * C * sizeof(float),
+                          ncol * C * sizeof(float), hipMemcpyDeviceToHost);
             if (e == hipSuccess && r->capture) {
                 r->counts.resize(GC);
                 e = hipMemcpy(r->counts.data(), (char*)S->arena + r->off_counts, GC * sizeof(int32_t),
@@ -276,6 +298,111 @@ void finish(scrna_server* S, std::vector<Req*>& done) {
         }
     }
     S->cv_done.notify_all();
+}
+
+// One launch of the stepper within a round: tiles of one kernel shape.
+struct RoundGroup {
+    int CH = 0, NW = 0, Smax = 1;
+    bool multi = false;                    // requests advancing several chunks this round
+    int64_t min_steps = INT64_MAX;         // fewest steps any of its tiles takes
+    std::vector<Tile> tiles;
+};
+
+// Advances every request in `run` by its steps for this round and waits for
+// the device.  Updates s_done; returns the first device error.
+constexpr double kSegmentBelowFill = 0.85;
+
+hipError_t run_round(scrna_server* S, const std::vector<Req*>& run) {
+    constexpr int kMaxChunks = 16;
+    double cmax = 0.0;
+    for (Req* r : run) cmax = std::max(cmax, r->cost);
+    std::map<std::tuple<bool, int, int>, RoundGroup> groups;   // multi-chunk groups sort first
+    for (Req* r : run) {
+        const int k = r->cost > 0.0 ? (int)std::min<double>(kMaxChunks, std::max(1.0, std::floor(cmax / r->cost))) : 1;
+        const int64_t s0 = r->s_done, s1 = std::min<int64_t>(r->cn.n_steps, s0 + (int64_t)S->chunk * k);
+        RoundGroup& g = groups[std::make_tuple(k == 1, r->cn.CH, r->cn.NW)];
+        g.CH = r->cn.CH;
+        g.NW = r->cn.NW;
+        g.multi = k > 1;
+        g.Smax = std::max(g.Smax, r->cn.S);
+        g.min_steps = std::min<int64_t>(g.min_steps, s1 - s0);
+        for (int64_t c0 = 0; c0 < r->cn.cells; c0 += 64) {
+            Tile t{};
+            t.blob = (int32_t)r->off_blob;
+            t.cell0 = (int32_t)c0;
+            t.ncell = (int32_t)std::min<int64_t>(64, r->cn.cells - c0);
+            t.net = 0;
+            t.s0 = (int32_t)s0;
+            t.s1 = (int32_t)s1;
+            g.tiles.push_back(t);
+        }
+        r->s_done = s1;
+    }
+    // One tile table and one schedule area for the round.
+    std::vector<Tile> all;
+    std::vector<size_t> tile0, sched0;
+    std::vector<int> nseg;
+    size_t sched_total = 0;
+    for (auto& kv : groups) {
+        RoundGroup& g = kv.second;
+        tile0.push_back(all.size());
+        all.insert(all.end(), g.tiles.begin(), g.tiles.end());
+        const int n_tiles = (int)g.tiles.size();
+        const int nbuf = stepper_lds_bytes(g.Smax, 2) <= S->ctx->lds_limit_soft ? 2 : 1;
+        const long long resident =
+            (long long)stepper_blocks_per_cu(g.CH, g.NW, stepper_lds_bytes(g.Smax, nbuf)) * S->ctx->num_cu;
+        // Segment only when the unsegmented launch would leave its last wave
+        // of resident workgroups mostly empty: rounds are short, so a
+        // segment's own start and hand-over cost more than in a whole-horizon plan.
+        int k = 1;
+        if (!g.multi && resident > 0) {
+            const long long waves = ((long long)n_tiles + resident - 1) / resident;
+            if ((double)n_tiles < kSegmentBelowFill * (double)(waves * resident))
+                k = choose_segments(n_tiles, resident, g.min_steps, 16);
+        }
+        nseg.push_back(k);
+        sched0.push_back(sched_total);
+        if (k > 1) sched_total += sched_words(n_tiles);
+    }
+    hipError_t e = hipSuccess;
+    if (!ensure_dev(S, (void**)&S->d_tiles, &S->tiles_cap, all.size(), sizeof(Tile))) return hipErrorOutOfMemory;
+    if (sched_total && !ensure_dev(S, (void**)&S->d_sched, &S->sched_cap, sched_total, sizeof(int)))
+        return hipErrorOutOfMemory;
+    e = hipMemcpyAsync(S->d_tiles, all.data(), all.size() * sizeof(Tile), hipMemcpyHostToDevice, S->stream);
+    if (e == hipSuccess) e = hipEventRecord(S->ev_tiles, S->stream);
+    int gi = 0;
+    for (auto& kv : groups) {
+        if (e != hipSuccess) break;
+        const RoundGroup& g = kv.second;
+        hipStream_t st = gi % S->kStreams == 0 ? S->stream : S->side[gi % S->kStreams - 1];
+        if (st != S->stream) e = hipStreamWaitEvent(st, S->ev_tiles, 0);
+        const int nbuf = stepper_lds_bytes(g.Smax, 2) <= S->ctx->lds_limit_soft ? 2 : 1;
+        int* sched = nullptr;
+        if (e == hipSuccess && nseg[gi] > 1) {
+            sched = S->d_sched + sched0[gi];
+            e = launch_sched_reset(sched, (int)sched_words((int)g.tiles.size()), st);
+        }
+        if (e == hipSuccess)
+            e = launch_stepper(g.CH, g.NW, (int)g.tiles.size(), S->blob_arena, S->d_tiles + tile0[gi],
+                               (float*)S->arena, nullptr, (uint8_t*)S->arena, nullptr, nullptr, 0, INT32_MAX, nbuf,
+                               stepper_lds_bytes(g.Smax, nbuf), sched, nseg[gi], S->wait_ticks, st);
+        ++gi;
+    }
+    for (int i = 0; i < S->kStreams - 1; ++i) {
+        const hipError_t es = hipStreamSynchronize(S->side[i]);
+        if (e == hipSuccess) e = es;
+    }
+    {
+        const hipError_t es = hipStreamSynchronize(S->stream);
+        if (e == hipSuccess) e = es;
+    }
+    for (size_t i = 0; e == hipSuccess && i < nseg.size(); ++i) {
+        if (nseg[i] <= 1) continue;
+        int err = 0;
+        e = hipMemcpy(&err, S->d_sched + sched0[i] + SCHED_ERROR, sizeof(int), hipMemcpyDeviceToHost);
+        if (e == hipSuccess && err) e = hipErrorLaunchTimeOut;
+    }
+    return e;
 }
 
 void worker_main(scrna_server* S) {
@@ -323,48 +450,10 @@ void worker_main(scrna_server* S) {
             continue;
         }
         const auto t0 = Clock::now();
-        // One round: every active request advances by up to `chunk` steps.
-        std::map<std::pair<int, int>, std::vector<Tile>> groups;
-        std::map<std::pair<int, int>, int> gS;
+        hipError_t e = run_round(S, run);
         std::vector<Req*> finished;
-        for (Req* r : run) {
-            const int64_t s0 = r->s_done, s1 = std::min<int64_t>(r->cn.n_steps, s0 + S->chunk);
-            auto key = std::make_pair(r->cn.CH, r->cn.NW);
-            auto& v = groups[key];
-            gS[key] = std::max(gS[key], r->cn.S);
-            for (int64_t c0 = 0; c0 < r->cn.cells; c0 += 64) {
-                Tile t{};
-                t.blob = (int32_t)r->off_blob;
-                t.cell0 = (int32_t)c0;
-                t.ncell = (int32_t)std::min<int64_t>(64, r->cn.cells - c0);
-                t.net = 0;
-                t.s0 = (int32_t)s0;
-                t.s1 = (int32_t)s1;
-                v.push_back(t);
-            }
-            r->s_done = s1;
-            if (s1 >= r->cn.n_steps) finished.push_back(r);
-        }
-        std::vector<Tile> all;
-        std::vector<std::pair<std::pair<int, int>, std::pair<int, int>>> spans;
-        for (auto& kv : groups) {
-            spans.push_back({kv.first, {(int)all.size(), (int)kv.second.size()}});
-            all.insert(all.end(), kv.second.begin(), kv.second.end());
-        }
-        hipError_t e = hipSuccess;
-        if (!ensure_dev(S, (void**)&S->d_tiles, &S->tiles_cap, all.size(), sizeof(Tile))) e = hipErrorOutOfMemory;
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(S->d_tiles, all.data(), all.size() * sizeof(Tile), hipMemcpyHostToDevice, S->stream);
-        for (auto& sp : spans) {
-            if (e != hipSuccess) break;
-            const int CH = sp.first.first, NW = sp.first.second;
-            const int Smax = std::max(1, gS[sp.first]);
-            const int nbuf = stepper_lds_bytes(Smax, 2) <= S->ctx->lds_limit_soft ? 2 : 1;
-            e = launch_stepper(CH, NW, sp.second.second, S->blob_arena, S->d_tiles + sp.second.first,
-                               (float*)S->arena, nullptr, (uint8_t*)S->arena, nullptr, nullptr, 0, INT32_MAX, nbuf,
-                               stepper_lds_bytes(Smax, nbuf), nullptr, 1, 0, S->stream);
-        }
-        if (e == hipSuccess) e = hipStreamSynchronize(S->stream);
+        for (Req* r : run)
+            if (r->s_done >= r->cn.n_steps) finished.push_back(r);
         if (e != hipSuccess) {
             std::lock_guard<std::mutex> lk(S->mu);
             for (Req* r : run) {
@@ -396,6 +485,24 @@ void worker_main(scrna_server* S) {
     }
 }
 
+void destroy_streams(scrna_server* S) {
+    if (S->stream) {
+        (void)hipStreamSynchronize(S->stream);
+        (void)hipStreamDestroy(S->stream);
+        S->stream = nullptr;
+    }
+    for (hipStream_t& st : S->side) {
+        if (!st) continue;
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+        st = nullptr;
+    }
+    if (S->ev_tiles) {
+        (void)hipEventDestroy(S->ev_tiles);
+        S->ev_tiles = nullptr;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -422,11 +529,17 @@ int scrna_server_create(scrna_ctx* ctx, int32_t max_slots, scrna_server** out) {
     }
     S->arena_alloc.reset(new RangeAlloc(S->arena_bytes));
     S->blob_alloc.reset(new RangeAlloc(S->blob_words * 4));
-    if (hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) != hipSuccess) {
+    bool ok = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) == hipSuccess;
+    for (int i = 0; ok && i < S->kStreams - 1; ++i)
+        ok = hipStreamCreateWithFlags(&S->side[i], hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&S->ev_tiles, hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        destroy_streams(S.get());
         ctx_free(ctx, S->arena, S->arena_bytes);
         ctx_free(ctx, S->blob_arena, S->blob_words * 4);
         return set_error(SCRNA_ERR_DEVICE, "hipStreamCreate failed");
     }
+    S->wait_ticks = 60000LL * ctx->wall_khz;   // 60 s bound on a segment's wait for its predecessor
     scrna_server* raw = S.get();
     S->worker = std::thread(worker_main, raw);
     *out = S.release();
@@ -478,6 +591,12 @@ int scrna_wait(scrna_server* S, int64_t ticket, int64_t timeout_ms, scrna_output
     if (timeout_ms < 0) S->cv_done.wait(lk, ready);
     else if (!S->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
         return set_error(SCRNA_ERR_TIMEOUT, "request %lld still running", (long long)ticket);
+    // Take the finished request out of the table and copy it out unlocked,
+    // so that the worker is not held up by the conversion.
+    std::unique_ptr<Req> own = std::move(it->second);
+    S->reqs.erase(it);
+    lk.unlock();
+    r = own.get();
     int rc = SCRNA_OK;
     if (r->st == St::Cancelled) rc = set_error(SCRNA_ERR_CANCELLED, "request %lld was cancelled", (long long)ticket);
     else if (r->st == St::Failed) rc = set_error(r->rc, "%s", r->err.c_str());
@@ -488,9 +607,12 @@ int scrna_wait(scrna_server* S, int64_t ticket, int64_t timeout_ms, scrna_output
             rc = set_error(SCRNA_ERR_INVALID, "output leading dimension too small");
         } else {
             if (out->y) {
-                const size_t ncol = r->y.size() / C;
-                for (size_t col = 0; col < ncol; ++col)
-                    std::memcpy(out->y + col * ld, r->y.data() + col * C, C * sizeof(double));
+                const size_t ncol = C ? r->y.size() / C : 0;
+                for (size_t col = 0; col < ncol; ++col) {
+                    const float* src = r->y.data() + col * C;
+                    double* dst = out->y + col * ld;
+                    for (size_t c = 0; c < C; ++c) dst[c] = (double)src[c];
+                }
             }
             if (out->counts && r->capture)
                 for (size_t g = 0; g < G; ++g)
@@ -498,7 +620,6 @@ int scrna_wait(scrna_server* S, int64_t ticket, int64_t timeout_ms, scrna_output
             if (out->moments) std::memcpy(out->moments, r->moments.data(), G * 5 * sizeof(double));
         }
     }
-    S->reqs.erase(it);
     return rc;
 }
 
@@ -553,11 +674,9 @@ int scrna_server_destroy(scrna_server* S) {
     S->cv_work.notify_all();
     if (S->worker.joinable()) S->worker.join();
     (void)hipSetDevice(S->ctx->device);
-    if (S->stream) {
-        (void)hipStreamSynchronize(S->stream);
-        (void)hipStreamDestroy(S->stream);
-    }
+    destroy_streams(S);
     if (S->d_tiles) (void)hipFree(S->d_tiles);
+    if (S->d_sched) (void)hipFree(S->d_sched);
     if (S->d_mjobs) (void)hipFree(S->d_mjobs);
     if (S->d_cjobs) (void)hipFree(S->d_cjobs);
     ctx_free(S->ctx, S->arena, S->arena_bytes);

@@ -49,6 +49,30 @@ def test_mix_equals_alone(ctx):
         assert st["p99_ms"] >= st["p50_ms"] > 0
 
 
+def test_mix_with_segmented_and_multichunk_rounds(ctx):
+    """A request whose tiles fill the device runs segmented launches; small
+    requests beside it advance several chunks per round, in launches on other
+    streams.  Each result still equals the request run alone."""
+    rng = np.random.default_rng(33)
+    reqs = [dict(G=50, cells=60000, T=3.0, tele=True), dict(G=10, cells=500, T=25.0, tele=False),
+            dict(G=20, cells=3000, T=12.0, tele=True), dict(G=10, cells=70, T=7.3, tele=True)]
+    for i, r in enumerate(reqs):
+        r["net"] = sc.samplers.mixed_network(r["G"], rng, n_tf=(2, 4), targets_per_tf=(2, 5), n_and=(0, 2),
+                                             n_or=(0, 2), telegraph=r["tele"])
+        r["cap"] = sc.CaptureModel(0.1, 0.3, "mrna") if i % 2 == 0 else None
+    alone = [sc.simulate(r["net"], sc.CLE(0.01), KIN, cell_num=r["cells"], T=r["T"], readout="protein",
+                         capture=r["cap"], seed=77 + i, full=True, ctx=ctx) for i, r in enumerate(reqs)]
+    with sc.Server(max_slots=8, ctx=ctx) as srv:
+        tickets = [srv.submit(r["net"], sc.CLE(0.01), KIN, cell_num=r["cells"], T=r["T"], readout="protein",
+                              capture=r["cap"], seed=77 + i) for i, r in enumerate(reqs)]
+        for i, t in enumerate(tickets):
+            got = srv.wait(t)
+            np.testing.assert_array_equal(got.y, alone[i].y)
+            if reqs[i]["cap"] is not None:
+                np.testing.assert_array_equal(got.counts, alone[i].counts)
+            np.testing.assert_allclose(got.moments[0], alone[i].moments[0], rtol=1e-12)
+
+
 def test_cancel_and_budget(ctx):
     net = sc.samplers.mixed_network(20, np.random.default_rng(1), n_tf=(2, 3), targets_per_tf=(2, 4),
                                     n_and=(0, 1), n_or=(0, 1))
