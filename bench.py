@@ -66,8 +66,11 @@ def valu_census(net, dense: bool = False) -> dict:
         square root folds into the update's two, sqrt(r^2 * var));
       Philox4x32-10: per round two 32x32-bit products and two 3-input xors
         (v_bitop3_b32); the key schedule is wave-uniform, scalar work.  So 20
-        products + 20 integer ops per call, 0.5 call per gene (+0.25 with
-        telegraph switching, which adds 3 integer ops).
+        products + 20 integer ops per call, 0.5 call per gene;
+      telegraph switching: 6 integer ops per gene (its 18 switching bits are
+        the spare low bits of its two normal words: 3 ops to join them, 1
+        threshold select, 2 compares); the extra draw for the 2^-18 of cells
+        whose 18 bits do not decide is not counted.
     Returns the class counts, the FMA-equivalent total and the FP32-equivalent
     flops (2 per FMA-equivalent op).
     """
@@ -77,9 +80,9 @@ def valu_census(net, dense: bool = False) -> dict:
     d_add = 0.0 if dense else np.count_nonzero(A) / G
     inc = sum(len(e.sources) for e in list(net.cooperative) + list(net.redundant)) / G
     tele = bool(np.any(np.isfinite(net.k_on)))
-    calls = 0.5 + (0.25 if tele else 0.0)
+    calls = 0.5
     counts = {"fp32": 33 + 2 * d_add + 3 * inc + 5, "trans": 3 + 3,
-              "int": 20 * calls + (3 if tele else 0), "mul32": 20 * calls}
+              "int": 20 * calls + (6 if tele else 0), "mul32": 20 * calls}
     fma_eq = (counts["fp32"] * COST_FP32 + counts["trans"] * COST_TRANS + counts["int"] * COST_INT
               + counts["mul32"] * COST_MUL32)
     return {"counts": {k: round(v, 3) for k, v in counts.items()}, "fma_equiv_ops": fma_eq,

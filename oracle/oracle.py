@@ -62,6 +62,9 @@ def load():
         lib = C.CDLL(LIB)
         lib.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         lib.orc_normal_pair.argtypes = [C.c_uint32, C.c_uint32, _dp, _dp]
+        lib.orc_cle_flip.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32,
+                                     C.c_int32, C.c_uint32]
+        lib.orc_cle_flip.restype = C.c_int
         lib.orc_regulation_matrix.argtypes = [C.POINTER(OrcNetwork), C.POINTER(OrcKinetics), _dp, _dp]
         lib.orc_regulation_loop.argtypes = [C.POINTER(OrcNetwork), C.POINTER(OrcKinetics), _dp, _dp]
         lib.orc_cle_simulate.argtypes = [C.POINTER(OrcNetwork), C.POINTER(OrcKinetics), C.POINTER(OrcRun),
@@ -131,6 +134,11 @@ def philox(ctr, key):
     o = (C.c_uint32 * 4)()
     lib.orc_philox4x32_10(c, k, o)
     return list(o)
+
+
+def cle_flip(a, b, thr, seed, cell, step, g, c3base=0):
+    """CLE promoter switch decision (1 = flip) of gene g from its normal words a, b."""
+    return int(load().orc_cle_flip(a, b, thr, seed, cell, step, g, c3base))
 
 
 def normal_pair(a, b):

@@ -43,8 +43,10 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 
 /* Stream purposes (low 4 bits of counter word 3; the net id fills the rest). */
 enum {
-    ORC_P_NORMAL = 0,      /* ctr = (cell, step, gene>>1): two genes' (xi_m, xi_p) */
-    ORC_P_SWITCH = 1,      /* ctr = (cell, step, gene>>2): one u32 per gene */
+    ORC_P_NORMAL = 0,      /* ctr = (cell, step, gene>>1): two genes' (xi_m, xi_p); low 9 bits of
+                              each word: the gene's CLE switching bits (orc_cle_flip) */
+    ORC_P_SWITCH = 1,      /* ctr = (cell, step, gene>>2): one u32 per gene (tau-leap switching;
+                              14 extension bits of the CLE switch) */
     ORC_P_PROM_INIT = 2,   /* ctr = (cell, 0, gene>>2) */
     ORC_P_CAPTURE_EFF = 3, /* ctr = (cell, 0, 0) */
     ORC_P_CAPTURE_BIN = 4, /* ctr = (cell, gene, block) */
@@ -229,6 +231,7 @@ static void orc_promoter_init(const orc_tele* t, int G, uint64_t seed, uint32_t 
     }
 }
 
+/* Tau-leap switching: one u32 of the switch block per gene. */
 static void orc_promoter_switch(const orc_tele* t, int G, uint64_t seed, uint32_t c3base, uint32_t cell,
                                 uint32_t step, uint8_t* on) {
     uint32_t u[4];
@@ -240,6 +243,23 @@ static void orc_promoter_switch(const orc_tele* t, int G, uint64_t seed, uint32_
         if (on[g]) { if (x < t->thr_off[g]) on[g] = 0; }
         else       { if (x < t->thr_on[g])  on[g] = 1; }
     }
+}
+
+/* Promoter switch of gene g in the CLE steppers (tauleap_binomial.jl:110-121):
+ * flip iff the 32-bit switching uniform X is below the threshold of the
+ * current state.  X = (a & 511) << 23 | (b & 511) << 14 | e, where a and b are
+ * the gene's radius and angle words of this step's normal block (the
+ * Box-Muller transform reads only their top 23 bits) and e is the top 14 bits
+ * of the gene's word of the switch block (cell, step, g >> 2), which is drawn
+ * only when the top 18 bits of X leave the comparison open. */
+int orc_cle_flip(uint32_t a, uint32_t b, uint64_t thr, uint64_t seed, uint32_t cell, uint32_t step, int32_t g,
+                 uint32_t c3base) {
+    const uint64_t x = ((uint64_t)(a & 511u) << 23) | ((uint64_t)(b & 511u) << 14);
+    if (x + 16384u <= thr) return 1;
+    if (x >= thr) return 0;
+    uint32_t u[4];
+    orc_draw(seed, cell, step, (uint32_t)(g >> 2), c3base | ORC_P_SWITCH, u);
+    return (x | (uint64_t)(u[g & 3] >> 18)) < thr;
 }
 
 /* ---------------------------------------------------------------------- */
@@ -294,19 +314,23 @@ int orc_cle_simulate(const orc_network* net, const orc_kinetics* kin, const orc_
         orc_promoter_init(&tele, G, run->seed, c3base, (uint32_t)c, on);
         int64_t snap = 0;
         for (int64_t s = 0; s < run->n_steps; ++s) {
-            if (tele.telegraph_any)
-                orc_promoter_switch(&tele, G, run->seed, c3base, (uint32_t)c, (uint32_t)s, on);
             orc_regulation_matrix(net, kin, P, reg);                    /* cle.jl:79-106 */
-            uint32_t u[4];
+            uint32_t u[4] = {0, 0, 0, 0};
             for (int g = 0; g < G; ++g) {
                 double xm, xp;
+                const int o = (g & 1) * 2;
+                if ((g & 1) == 0 && (!run->injected || tele.telegraph_any))
+                    orc_draw(run->seed, (uint32_t)c, (uint32_t)s, (uint32_t)(g >> 1), c3base | ORC_P_NORMAL, u);
+                /* switching first; the regulation above does not read promoter states */
+                if (tele.telegraph_any && tele.switching[g] &&
+                    orc_cle_flip(u[o], u[o + 1], on[g] ? tele.thr_off[g] : tele.thr_on[g], run->seed,
+                                 (uint32_t)c, (uint32_t)s, g, c3base))
+                    on[g] ^= 1;
                 if (run->injected) {
                     size_t base = ((size_t)s * G + g) * 2;
                     xm = run->injected[(base + 0) * (size_t)run->cell_num + c];
                     xp = run->injected[(base + 1) * (size_t)run->cell_num + c];
                 } else {
-                    if ((g & 1) == 0) orc_draw(run->seed, (uint32_t)c, (uint32_t)s, (uint32_t)(g >> 1), c3base | ORC_P_NORMAL, u);
-                    int o = (g & 1) * 2;
                     orc_normal_pair(u[o], u[o + 1], &xm, &xp);
                 }
                 /* telegraph gating of the whole transcription propensity
@@ -655,20 +679,23 @@ int orc_cle_population(const orc_network* net, const orc_kinetics* kin, const or
     for (int64_t s = 0; s < run->n_steps; ++s) {
         for (int64_t c = 0; c < N; ++c) {
             double* Mc = M + c * G; double* Pc = P + c * G; uint8_t* oc = on + c * G;
-            if (tele.telegraph_any)
-                orc_promoter_switch(&tele, G, run->seed, c3base, (uint32_t)c, (uint32_t)s, oc);
             for (int g = 0; g < G; ++g) pcol[g] = Pc[g];
             orc_regulation_matrix(net, kin, pcol, reg);                 /* cle.jl:190-217 */
-            uint32_t u[4];
+            uint32_t u[4] = {0, 0, 0, 0};
             for (int g = 0; g < G; ++g) {
                 double xm, xp;
+                const int o = (g & 1) * 2;
+                if ((g & 1) == 0 && (!run->injected || tele.telegraph_any))
+                    orc_draw(run->seed, (uint32_t)c, (uint32_t)s, (uint32_t)(g >> 1), c3base | ORC_P_NORMAL, u);
+                if (tele.telegraph_any && tele.switching[g] &&
+                    orc_cle_flip(u[o], u[o + 1], oc[g] ? tele.thr_off[g] : tele.thr_on[g], run->seed,
+                                 (uint32_t)c, (uint32_t)s, g, c3base))
+                    oc[g] ^= 1;
                 if (run->injected) {
                     size_t base = ((size_t)s * G + g) * 2;
                     xm = run->injected[(base + 0) * (size_t)N + c];
                     xp = run->injected[(base + 1) * (size_t)N + c];
                 } else {
-                    if ((g & 1) == 0) orc_draw(run->seed, (uint32_t)c, (uint32_t)s, (uint32_t)(g >> 1), c3base | ORC_P_NORMAL, u);
-                    int o = (g & 1) * 2;
                     orc_normal_pair(u[o], u[o + 1], &xm, &xp);
                 }
                 double tx = oc[g] ? (reg[g] + net->basals[g]) * V[c] : 0.0;   /* cle.jl:224-225 */

@@ -93,6 +93,9 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 
 /* The Box-Muller pair used by the CLE stepper for two u32 inputs. */
 void orc_normal_pair(uint32_t a, uint32_t b, double* z0, double* z1);
+/* CLE promoter switch decision of gene g from its normal words a, b (1 = flip). */
+int orc_cle_flip(uint32_t a, uint32_t b, uint64_t thr, uint64_t seed, uint32_t cell, uint32_t step, int32_t g,
+                 uint32_t c3base);
 
 /* Regulation for one cell, matrix form (cle.jl:79-106) and loop form
  * (network.jl:158-193).  p: [G] proteins; reg: [G] output. */

@@ -71,6 +71,41 @@ __device__ __forceinline__ void hill_to_lds(const float (&p)[CH], uint32_t srcm,
     }
 }
 
+// Telegraph switch of one gene in the 64 cells of a wave
+// (tauleap_binomial.jl:110-121): a cell flips iff its 32-bit switching
+// uniform X is below the threshold of its current state (ON cells use the
+// ON->OFF one; thresholds are ceil(prob * 2^32) <= 2^32).  X = q << 14 | e:
+// q = the low 9 bits of the gene's radius word a and angle word b of this
+// step's normal block (Box-Muller reads only their top 23 bits), e = the top
+// 14 bits of the gene's word of the switch block (cell, step, gene >> 2).
+// With T = floor(threshold / 2^14) (t_on / t_off), q < T flips and q > T
+// does not, whatever e is; e is drawn only in the rare waves where some lane
+// has q == T (probability 2^-18 per cell).
+__device__ __forceinline__ uint64_t cle_switch(uint64_t on, uint32_t a, uint32_t b, uint32_t t_on, uint32_t t_off,
+                                               uint32_t cell, uint32_t step, int gene, uint32_t c3, uint32_t k0,
+                                               uint32_t k1, const uint32_t* __restrict__ blobs, uint32_t tw_off) {
+    const uint32_t q = ((a & 0x1FFu) << 9) | (b & 0x1FFu);
+    const bool cur = __builtin_amdgcn_inverse_ballot_w64(on);
+    const uint32_t T = cur ? t_off : t_on;
+    uint64_t flip = __builtin_amdgcn_ballot_w64(q < T);
+    const uint64_t amb = __builtin_amdgcn_ballot_w64(q == T);
+    if (amb != 0) {
+        // Here T < 2^18, so the threshold is below 2^32 and X fits in 32 bits.
+        // The threshold words' offset is made opaque inside the branch, and
+        // the keys too (round keys recomputed here), so that nothing of this
+        // path is hoisted into the step and kept live there.
+        uint32_t off = tw_off;
+        asm("" : "+s"(off) : "s"((uint32_t)amb));
+        const U4 u = philox4x32_10_uk(cell, step, (uint32_t)(gene >> 2), c3, k0, k1);
+        const int w = gene & 3;
+        const uint32_t e = (w == 0 ? u.x : w == 1 ? u.y : w == 2 ? u.z : u.w) >> 18;
+        const uint32_t* __restrict__ tw = blobs + off;
+        const uint32_t thr = cur ? tw[T_OFF_LO] : tw[T_ON_LO];
+        flip |= amb & __builtin_amdgcn_ballot_w64(((q << 14) | e) < thr);
+    }
+    return on ^ flip;
+}
+
 // Occupancy and register slots for up to 4 genes per lane.  The kernel asks
 // for 7 waves per SIMD (72 VGPRs, 94 SGPRs), which holds two register slots
 // per gene for in-edges without spilling.  On the G=20 benchmark network
@@ -144,8 +179,9 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
     uint64_t on[CH];
     // Per-gene constants of the step loop kept in VGPRs (uniform values; VALU
     // operands at no cost, and no scalar registers or reloads across the
-    // loop): the low words of the OFF->ON and ON->OFF thresholds (0 for genes
-    // that never switch, which then never flip) and the LDS byte address of the
+    // loop): the OFF->ON and ON->OFF thresholds in units of 2^14,
+    // floor(threshold / 2^14) <= 2^18 (0 for genes that never switch, which
+    // then never flip; see cle_switch), and the LDS byte address of the
     // gene's Hill row pair in this lane (source genes only).
     uint32_t thr_on[CH], thr_off[CH], haddr[CH];
     float beta_v[CH];                                      // basal rate of each gene
@@ -157,7 +193,6 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
     uint32_t eaddr[CH][EK > 0 ? EK : 1];
     float ew[CH][EK > 0 ? EK : 1];
     uint32_t sw_mask = 0;                                  // bit k: gene g0+k switches
-    uint32_t hi_mask = 0;                                  // bit 2k / 2k+1: ON / OFF threshold is 2^32
     uint32_t src_mask = 0;                                 // bit k: gene g0+k is a regulation source
     uint32_t extra_mask = 0;
     double V = 1.0;                                       // cell volume (population runs)
@@ -193,10 +228,8 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
                 if ((flags & F_GATES) && gate_off[g + 1] > gate_off[g]) extra_mask |= 1u << (16 + k);
                 const uint32_t* t = telew + g * 8;
                 if (tele && (t[T_FLAGS] & 1u)) {
-                    thr_on[k] = t[T_ON_LO];
-                    thr_off[k] = t[T_OFF_LO];
-                    hi_mask |= ((t[T_ON_HI] != 0u) ? 1u : 0u) << (2 * k);
-                    hi_mask |= ((t[T_OFF_HI] != 0u) ? 1u : 0u) << (2 * k + 1);
+                    thr_on[k] = t[T_ON_HI] != 0u ? 1u << 18 : t[T_ON_LO] >> 14;
+                    thr_off[k] = t[T_OFF_HI] != 0u ? 1u << 18 : t[T_OFF_LO] >> 14;
                 }
                 const int slot = slot_of[g];
                 if (slot >= 0) {
@@ -243,7 +276,6 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
     }
     if (pop) V = vol_pool[off_vol + cell];   // written by the population init kernel or a previous phase
     sw_mask = tele ? __builtin_amdgcn_readfirstlane(sw_mask) : 0u;
-    hi_mask = __builtin_amdgcn_readfirstlane(hi_mask);
     src_mask = __builtin_amdgcn_readfirstlane(src_mask);
     extra_mask = __builtin_amdgcn_readfirstlane(extra_mask);
 
@@ -258,10 +290,9 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
         // write and turn every later blob read into a vector load) and takes s
         // as an input so that it stays inside the loop; it works on the
         // offset, not on the pointer, which keeps the address space global.
-        uint32_t boff = blob_off, swm = sw_mask, him = hi_mask, srcm = src_mask, xtra = extra_mask;
+        uint32_t boff = blob_off, swm = sw_mask, srcm = src_mask, xtra = extra_mask;
         asm("" : "+s"(boff) : "s"(s));
         asm("" : "+s"(swm) : "s"(s));
-        asm("" : "+s"(him) : "s"(s));
         asm("" : "+s"(srcm) : "s"(s));
         asm("" : "+s"(xtra) : "s"(s));
         const uint32_t* __restrict__ Bs = blobs + boff;
@@ -272,36 +303,6 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
         // Buffer of this step: 2S Hill rows and a zero row.
         float* __restrict__ H = lds + (nbuf == 2 ? (s & 1) * (2 * (int)Bs[H_S] + 1) * 64 : 0);
         const uint32_t hbase = (uint32_t)((H - lds) * 4);
-
-        // Telegraph switching first (tauleap_binomial.jl:110-121).  A lane
-        // flips when its u32 is below the threshold of its current state
-        // (ON cells use the ON->OFF one); thresholds are at most 2^32, and one
-        // of 2^32 (hi_mask) flips every lane.  Genes that never switch have
-        // zero thresholds and never flip.
-        if (swm != 0u) {
-#pragma unroll
-            for (int k = 0; k < CH; k += 4) {
-                const int g = g0 + k;
-                if (k < nv) {
-                    const U4 u = philox4x32_10((uint32_t)cell, (uint32_t)s, (uint32_t)(g >> 2), netc | kSwitch, sk0,
-                                               sk1);
-                    const uint32_t uu[4] = {u.x, u.y, u.z, u.w};
-#pragma unroll
-                    for (int j = 0; j < 4 && k + j < CH; ++j) {
-                        const uint32_t x = uu[(g & 3) + j];
-                        const uint32_t thr =
-                            __builtin_amdgcn_inverse_ballot_w64(on[k + j]) ? thr_off[k + j] : thr_on[k + j];
-                        uint64_t flip = __builtin_amdgcn_ballot_w64(x < thr);
-                        if (him != 0u) {
-                            const uint64_t all_on = ((him >> (2 * (k + j))) & 1u) ? ~0ull : 0ull;
-                            const uint64_t all_off = ((him >> (2 * (k + j) + 1)) & 1u) ? ~0ull : 0ull;
-                            flip |= (on[k + j] & all_off) | (~on[k + j] & all_on);
-                        }
-                        on[k + j] ^= flip;
-                    }
-                }
-            }
-        }
 
         // Hill values of this wave's source genes -> LDS (cle.jl:79-82).
         {
@@ -332,7 +333,23 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
                     // radius squared, scaled by dt, and the two unit directions), so that
                     // xi*sqrt(var) below costs one square root, sqrt(r2*var).
                     float r2[2], c[2], sn[2];
-                    if (INJ && (fl & F_INJECT)) {
+                    // The normal block also carries the genes' switching bits, so
+                    // injected-noise runs draw it for telegraph networks too.
+                    const bool inj = INJ && (fl & F_INJECT);
+                    U4 u = {0u, 0u, 0u, 0u};
+                    if (!inj || swm != 0u)
+                        u = philox4x32_10((uint32_t)cell, (uint32_t)s, (uint32_t)(g >> 1), netc | kNormal, sk0, sk1);
+                    // Telegraph switching of the two genes (tauleap_binomial.jl:110-121),
+                    // ahead of their transcription below; the Hill values written
+                    // above do not depend on promoter states.
+#pragma unroll
+                    for (int j = 0; j < 2 && k + j < CH; ++j) {
+                        if (k + j < nv && ((swm >> (k + j)) & 1u))
+                            on[k + j] = cle_switch(on[k + j], j ? u.z : u.x, j ? u.w : u.y, thr_on[k + j],
+                                                   thr_off[k + j], (uint32_t)cell, (uint32_t)s, g + j, netc | kSwitch,
+                                                   sk0, sk1, blobs, boff + Bs[H_OFF_TELE] + (uint32_t)(g + j) * 8u);
+                    }
+                    if (inj) {
                         const int Gs = (int)Bs[H_G];
                         const size_t Cs = (size_t)Bs[H_CELLS];
                         const float* __restrict__ inj = inj_pool + (size_t)u64_at(Bs, H_INJ_LO);
@@ -345,8 +362,6 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
                             sn[j] = inj[(base + 1) * Cs + cell];
                         }
                     } else {
-                        const U4 u = philox4x32_10((uint32_t)cell, (uint32_t)s, (uint32_t)(g >> 1), netc | kNormal,
-                                                   sk0, sk1);
                         normal_pair_r2(u.x, u.y, neg2ln2dt, r2[0], c[0], sn[0]);
                         normal_pair_r2(u.z, u.w, neg2ln2dt, r2[1], c[1], sn[1]);
                     }

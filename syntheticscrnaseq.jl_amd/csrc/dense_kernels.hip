@@ -48,19 +48,10 @@ __device__ __forceinline__ uint32_t pick4(const U4& u, int i) {
 
 constexpr int kDenseMaxKC = 8;   // Gp <= 256
 
-// Cross-lane moves inside a quad of lanes as DPP operands of a v_mov (no LDS
-// round trip as with ds_bpermute).  quad_from<R>: lane j receives lane
-// (j - R) & 3's value; xor1: lane j receives lane j ^ 1's value.
+// Cross-lane move inside a quad of lanes as a DPP operand of a v_mov (no
+// LDS round trip as with ds_bpermute): lane j receives lane j ^ 1's value.
 __device__ __forceinline__ uint32_t dpp_xor1(uint32_t x) {
     return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
-}
-__device__ __forceinline__ uint32_t quad_from(uint32_t x, int r) {
-    switch (r & 3) {
-        case 1: return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x93, 0xF, 0xF, false);   // [3,0,1,2]
-        case 2: return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, false);   // [2,3,0,1]
-        case 3: return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x39, 0xF, 0xF, false);   // [1,2,3,0]
-        default: return x;
-    }
 }
 
 template <int MODE, int NB, bool TELE, bool INJECT>
@@ -129,7 +120,7 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
     bool gor[NB], gslow[NB], wave_gate[NB], wave_slow[NB];
     float beta[NB], bias[NB];
     uint32_t tflags[NB];
-    uint64_t thr_on[NB], thr_off[NB];
+    uint32_t thr_on[NB], thr_off[NB];     // floor(threshold / 2^14), see cle_switch
     float m[NB][4], p[NB][4];
     uint32_t on = 0;                       // promoter bit 4*b + v
 
@@ -157,8 +148,8 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
         bias[b] = gl[b] ? d_as_f(B[B[H_OFF_BIAS] + gc]) : 0.0f;
         slot[b] = gl[b] ? ((const int32_t*)(B + B[H_OFF_SLOT]))[g[b]] : -1;
         tflags[b] = gl[b] ? telew[g[b] * 8 + T_FLAGS] : 0u;
-        thr_on[b] = d_u64(telew, gc * 8 + T_ON_LO);
-        thr_off[b] = d_u64(telew, gc * 8 + T_OFF_LO);
+        thr_on[b] = telew[gc * 8 + T_ON_HI] != 0u ? 1u << 18 : telew[gc * 8 + T_ON_LO] >> 14;
+        thr_off[b] = telew[gc * 8 + T_OFF_HI] != 0u ? 1u << 18 : telew[gc * 8 + T_OFF_LO] >> 14;
         q0[b] = gl[b] ? gate_off[g[b]] : 0;
         q1[b] = gl[b] ? gate_off[g[b] + 1] : 0;
         gw[b] = 0.0f;
@@ -216,25 +207,6 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
 
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
-            // Telegraph switching first (tauleap_binomial.jl:110-121).  The switch
-            // block (cell, s, g>>2) serves the 4 genes of a lane quad; lane j of
-            // the quad draws the block of its element v = j and the quad rotates
-            // words so every lane gets its gene's word for each of its cells.
-            if (tele) {
-                const U4 u = philox4x32_10_uk((uint32_t)cell(l & 3), (uint32_t)s, (uint32_t)(g[b] >> 2), c3s, k0, k1);
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int j = l & 3;                            // receives from lane (j - r) & 3,
-                    const uint32_t send = pick4(u, (j + r) & 3);    // which drew element (j - r) & 3
-                    const uint32_t got = quad_from(send, r);
-                    const int bit = 4 * b + ((j - r) & 3);
-                    if (tflags[b] & 1u) {
-                        const bool cur = (on >> bit) & 1u;
-                        const bool flip = (uint64_t)got < (cur ? thr_off[b] : thr_on[b]);
-                        on ^= (uint32_t)flip << bit;
-                    }
-                }
-            }
             // Hill values of the lane's gene for its 4 cells -> LDS.  The n = 2
             // test sits outside the cell loop so that the n = 2 path never
             // evaluates log/exp (as a select, both sides would run).
@@ -290,7 +262,9 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
         // Elementwise update of element (b, v) (cle.jl:108-120).
         // Noise enters as xi_m = c*sqrt(r2), xi_p = sn*sqrt(r2) (Box-Muller radius
         // squared, scaled by dt), so xi*sqrt(var) costs one square root, sqrt(r2*var).
-        auto update = [&](const int b, const int v, float r2, float c, float sn) {
+        // wa, wb: the radius and angle words of element (b, v)'s normal block,
+        // whose low 9 bits are its switching bits.
+        auto update = [&](const int b, const int v, float r2, float c, float sn, uint32_t wa, uint32_t wb) {
             float reg = acc[b][v] + bias[b];
             // AND / OR (cle.jl:89-106).  Lanes without a register gate read a
             // valid Hill row with weight 0.
@@ -308,6 +282,28 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
                     reg = fmaf(d_as_f(gates[q + 1]), type == GATE_AND ? prod : 1.0f - prod, reg);
                     q += 2 + ns;
                 }
+            }
+            // Telegraph switching first (tauleap_binomial.jl:110-121), as in
+            // cle_switch with one gene per lane: X = q << 14 | e flips iff below
+            // the threshold of the current state; e (the top 14 bits of the
+            // gene's word of the switch block) only matters when q == T.
+            if (tele) {
+                const int bit = 4 * b + v;
+                const bool cur = (on >> bit) & 1u;
+                const bool sw = tflags[b] & 1u;
+                const uint32_t q = ((wa & 0x1FFu) << 9) | (wb & 0x1FFu);
+                const uint32_t T = cur ? thr_off[b] : thr_on[b];
+                bool flip = sw && q < T;
+                const bool amb = sw && q == T;
+                if (__builtin_amdgcn_ballot_w64(amb) != 0) {
+                    const U4 u = philox4x32_10_uk((uint32_t)cell(v), (uint32_t)s, (uint32_t)(g[b] >> 2), c3s, k0,
+                                                  k1);
+                    const uint32_t e = pick4(u, g[b]) >> 18;
+                    const int gc = gl[b] ? g[b] : 0;
+                    const uint64_t thr = d_u64(telew, gc * 8 + (cur ? T_OFF_LO : T_ON_LO));
+                    if (amb) flip = ((((uint64_t)q << 14) | e) < thr);
+                }
+                on ^= (uint32_t)flip << bit;
             }
             const bool o = (on >> (4 * b + v)) & 1u;
             const float tx = o ? reg + beta[b] : 0.0f;
@@ -328,8 +324,18 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
             if (inject) {
                 const size_t base = ((size_t)s * G + (gl[b] ? g[b] : 0)) * 2;
 #pragma unroll
-                for (int v = 0; v < 4; ++v)
-                    update(b, v, dt, inj_pool[off_inj + base * C + cell(v)], inj_pool[off_inj + (base + 1) * C + cell(v)]);
+                for (int v = 0; v < 4; ++v) {
+                    // the switching bits still come from the normal block
+                    uint32_t wa = 0u, wb = 0u;
+                    if (tele) {
+                        const U4 u = philox4x32_10_uk((uint32_t)cell(v), (uint32_t)s, (uint32_t)(g[b] >> 1), c3n, k0,
+                                                      k1);
+                        wa = (g[b] & 1) ? u.z : u.x;
+                        wb = (g[b] & 1) ? u.w : u.y;
+                    }
+                    update(b, v, dt, inj_pool[off_inj + base * C + cell(v)], inj_pool[off_inj + (base + 1) * C + cell(v)],
+                           wa, wb);
+                }
             } else {
                 // The normal block (cell, s, g>>1) carries genes g (words 0,1) and
                 // g^1 (words 2,3), i.e. lanes l and l^1, which own the same cells.
@@ -343,10 +349,12 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
                     const uint32_t got_a = dpp_xor1(par ? u.x : u.z);
                     const uint32_t got_b = dpp_xor1(par ? u.y : u.w);
                     float r2, c, sn;
-                    normal_pair_r2(par ? got_a : mine_a, par ? got_b : mine_b, neg2ln2dt, r2, c, sn);
-                    update(b, t, r2, c, sn);
-                    normal_pair_r2(par ? mine_a : got_a, par ? mine_b : got_b, neg2ln2dt, r2, c, sn);
-                    update(b, t + 2, r2, c, sn);
+                    const uint32_t ta = par ? got_a : mine_a, tb = par ? got_b : mine_b;
+                    normal_pair_r2(ta, tb, neg2ln2dt, r2, c, sn);
+                    update(b, t, r2, c, sn, ta, tb);
+                    const uint32_t ua = par ? mine_a : got_a, ub = par ? mine_b : got_b;
+                    normal_pair_r2(ua, ub, neg2ln2dt, r2, c, sn);
+                    update(b, t + 2, r2, c, sn, ua, ub);
                 }
             }
         }

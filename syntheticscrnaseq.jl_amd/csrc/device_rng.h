@@ -14,8 +14,10 @@
 namespace scrna {
 
 enum Purpose : uint32_t {
-    kNormal = 0,       // (cell, step, gene>>1): xi_m, xi_p for genes 2q and 2q+1
-    kSwitch = 1,       // (cell, step, gene>>2): one u32 per gene
+    kNormal = 0,       // (cell, step, gene>>1): xi_m, xi_p for genes 2q and 2q+1 (top 23 bits of
+                       // each word); the low 9 bits of a gene's two words are its CLE switching bits
+    kSwitch = 1,       // (cell, step, gene>>2): one u32 per gene (tau-leap switching; the CLE
+                       // steppers take 14 more switching bits from it when 18 do not decide)
     kPromInit = 2,     // (cell, 0, gene>>2)
     kCaptureEff = 3,   // (cell, 0, 0)
     kCaptureBin = 4,   // (cell, gene, block)

@@ -201,6 +201,32 @@ def test_telegraph_promoter_chain():
     assert abs(an.telegraph_mean(k_on / mu_m, k_off / mu_m, beta / mu_m) / want - 1) < (k_on + k_off) * dt
 
 
+def test_cle_switch_decision_is_the_32bit_comparison():
+    """The CLE switch decides flip iff X < threshold for the 32-bit uniform
+    X = (a & 511) << 23 | (b & 511) << 14 | e, with e the top 14 bits of the gene's
+    word of the switch block (Philox pinned by the KATs above).  Checked at every
+    boundary of the 18-bit early decisions and on thresholds whose answer hinges on e."""
+    rng = np.random.default_rng(0)
+    for seed, cell, step, g, c3 in [(99, 5, 7, 6, 0), ((7 << 32) | 3, 123456, 4999, 1, (3 << 4)), (1, 0, 0, 0, 16)]:
+        blk = orc.philox([cell, step, g >> 2, c3 | 1], [seed & 0xFFFFFFFF, seed >> 32])
+        e = blk[g & 3] >> 18
+        thrs = [0, 1, 16383, 16384, 16385, 123456789, (1 << 32) - 1, 1 << 32,
+                (5 << 14) + e, (5 << 14) + e + 1, ((1 << 18) - 1 << 14) + e + 1]
+        n_rare = 0
+        for thr in thrs:
+            T = thr >> 14
+            for q in sorted({0, 1, 5, T - 1, T, T + 1, (1 << 18) - 1}):
+                if not 0 <= q < (1 << 18):
+                    continue
+                hi = rng.integers(0, 1 << 23, size=2)
+                a = int(hi[0]) << 9 | (q >> 9)
+                b = int(hi[1]) << 9 | (q & 511)
+                want = ((q << 14) | e) < thr
+                n_rare += (q << 14) < thr < ((q + 1) << 14)
+                assert orc.cle_flip(a, b, thr, seed, cell, step, g, c3) == want, (seed, thr, q)
+        assert n_rare >= 3          # decisions that read e were exercised
+
+
 def test_telegraph_closed_form_kats():
     """README.md:117-120 and test_telegraph_distribution.jl:39-58."""
     assert an.telegraph_mean(2.0, 5.0, 70.0) == pytest.approx(20.0)
