@@ -537,6 +537,17 @@ void plan_release(scrna_plan* p) {
     if (p->ev_start) (void)hipEventDestroy(p->ev_start);
     if (p->ev_stop) (void)hipEventDestroy(p->ev_stop);
     p->ev_start = p->ev_stop = nullptr;
+    for (int i = 0; i < scrna_plan::kSideStreams; ++i) {
+        if (p->side[i]) {
+            (void)hipStreamSynchronize(p->side[i]);
+            (void)hipStreamDestroy(p->side[i]);
+        }
+        if (p->ev_join[i]) (void)hipEventDestroy(p->ev_join[i]);
+        p->side[i] = nullptr;
+        p->ev_join[i] = nullptr;
+    }
+    if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+    p->ev_fork = nullptr;
     for (auto& u : p->unfused) unfused_release(u);
     if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
@@ -848,6 +859,34 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
         plan_release(P.get());
         return set_error(SCRNA_ERR_DEVICE, "hipEventCreate failed");
     }
+    // Stream lanes for the launch groups: largest work first, each to the
+    // least loaded lane, so that small groups fill the device beside the
+    // large ones instead of running after them with a mostly idle tail.
+    {
+        std::vector<int> order(P->groups.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+        auto cost = [&](int i) {
+            const LaunchGroup& g = P->groups[i];
+            return (double)g.n_tiles * (double)g.NW * (double)std::max(1, g.CH);
+        };
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+        P->n_lanes = (int)std::min<size_t>(1 + scrna_plan::kSideStreams, P->groups.size());
+        std::vector<double> load((size_t)std::max(1, P->n_lanes), 0.0);
+        for (int i : order) {
+            const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            P->groups[i].lane = l;
+            load[l] += cost(i);
+        }
+        bool ok = true;
+        for (int i = 0; ok && i + 1 < P->n_lanes; ++i)
+            ok = hipStreamCreateWithFlags(&P->side[i], hipStreamNonBlocking) == hipSuccess &&
+                 hipEventCreateWithFlags(&P->ev_join[i], hipEventDisableTiming) == hipSuccess;
+        if (ok && P->n_lanes > 1) ok = hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming) == hipSuccess;
+        if (!ok) {
+            plan_release(P.get());
+            return set_error(SCRNA_ERR_DEVICE, "stream or event creation failed");
+        }
+    }
     for (auto& cn : P->nets) P->cell_gene_steps += (double)cn.G * (double)cn.cells * (double)cn.n_steps;
     *out = P.release();
     return SCRNA_OK;
@@ -876,27 +915,36 @@ int choose_segments(int n_tiles, long long resident, int64_t steps, int max_k) {
 // A tile resumes from the state pool exactly when its first step is > 0.
 // `segmented` enables the segmented schedule of the FP32 and tau-leap stepper groups.
 int launch_steps(scrna_plan* P, hipStream_t st, int s_begin, int s_end, bool segmented) {
+    if (P->n_lanes > 1) {
+        HIP_TRY(hipEventRecord(P->ev_fork, st));
+        for (int i = 0; i + 1 < P->n_lanes; ++i) HIP_TRY(hipStreamWaitEvent(P->side[i], P->ev_fork, 0));
+    }
     for (const LaunchGroup& lg : P->groups) {
+        hipStream_t gs = lg.lane == 0 ? st : P->side[lg.lane - 1];
         if (lg.kind == 0 || lg.kind == 2) {
             int* sched = nullptr;
             int nseg = 1;
             if (segmented && lg.nseg > 1) {
                 sched = P->d_sched + lg.sched_off;
                 nseg = lg.nseg;
-                HIP_TRY(launch_sched_reset(sched, (int)sched_words(lg.n_tiles), st));
+                HIP_TRY(launch_sched_reset(sched, (int)sched_words(lg.n_tiles), gs));
             }
             if (lg.kind == 0)
                 HIP_TRY(launch_stepper(lg.CH, lg.NW, lg.n_tiles, P->d_blobs, P->d_tiles + lg.tile0, P->d_state,
                                        P->d_snap, P->d_prom, P->d_vol, P->d_inj, s_begin, s_end, lg.nbuf, lg.lds,
-                                       sched, nseg, P->wait_ticks, st));
+                                       sched, nseg, P->wait_ticks, gs));
             else
                 HIP_TRY(launch_tauleap(lg.CH, lg.NW, lg.n_tiles, P->d_blobs, P->d_tiles + lg.tile0, P->d_state,
                                        P->d_snap, P->d_prom, P->d_vol, s_begin, s_end, lg.nbuf, lg.lds, sched,
-                                       nseg, P->wait_ticks, st));
+                                       nseg, P->wait_ticks, gs));
         } else
             HIP_TRY(launch_dense(lg.mode, lg.tele, lg.inject, lg.NW, lg.n_tiles, P->d_blobs, P->d_tiles + lg.tile0,
                                  P->d_wpool, P->d_state, P->d_snap, P->d_prom, P->d_inj, s_begin, s_end, lg.lds,
-                                 st));
+                                 gs));
+    }
+    for (int i = 0; i + 1 < P->n_lanes; ++i) {
+        HIP_TRY(hipEventRecord(P->ev_join[i], P->side[i]));
+        HIP_TRY(hipStreamWaitEvent(st, P->ev_join[i], 0));
     }
     return SCRNA_OK;
 }

@@ -76,6 +76,7 @@ struct LaunchGroup {
     size_t lds = 0;
     int nseg = 1;                    // FP32 stepper: segments per tile on whole-horizon launches
     size_t sched_off = 0;            // its schedule block in scrna_plan::d_sched
+    int lane = 0;                    // stream lane: 0 = the launch stream, k > 0 = scrna_plan::side[k-1]
 };
 
 }  // namespace scrna
@@ -110,6 +111,12 @@ struct scrna_plan {
     int64_t max_cap_elems = 0;
     int max_mom_G = 0;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    // Launch groups run side by side: lanes > 0 fork from the launch stream
+    // at ev_fork and join back through ev_join (also inside graph capture).
+    static constexpr int kSideStreams = 3;
+    int n_lanes = 1;
+    hipStream_t side[kSideStreams] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[kSideStreams] = {};
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     // population buffers: daughters, claims, initial volume ranges
