@@ -81,11 +81,10 @@ __device__ __forceinline__ void hill_to_lds(const float (&p)[CH], uint32_t srcm,
 // With T = floor(threshold / 2^14) (t_on / t_off), q < T flips and q > T
 // does not, whatever e is; e is drawn only in the rare waves where some lane
 // has q == T (probability 2^-18 per cell).
-__device__ __forceinline__ uint64_t cle_switch(uint64_t on, uint32_t a, uint32_t b, uint32_t t_on, uint32_t t_off,
+__device__ __forceinline__ uint64_t cle_switch(bool cur, uint32_t a, uint32_t b, uint32_t t_on, uint32_t t_off,
                                                uint32_t cell, uint32_t step, int gene, uint32_t c3, uint32_t k0,
                                                uint32_t k1, const uint32_t* __restrict__ blobs, uint32_t tw_off) {
     const uint32_t q = ((a & 0x1FFu) << 9) | (b & 0x1FFu);
-    const bool cur = __builtin_amdgcn_inverse_ballot_w64(on);
     const uint32_t T = cur ? t_off : t_on;
     uint64_t flip = __builtin_amdgcn_ballot_w64(q < T);
     const uint64_t amb = __builtin_amdgcn_ballot_w64(q == T);
@@ -103,8 +102,34 @@ __device__ __forceinline__ uint64_t cle_switch(uint64_t on, uint32_t a, uint32_t
         const uint32_t thr = cur ? tw[T_OFF_LO] : tw[T_ON_LO];
         flip |= amb & __builtin_amdgcn_ballot_w64(((q << 14) | e) < thr);
     }
-    return on ^ flip;
+    return flip;                                   // lanes whose promoter flips
 }
+
+// Promoter states of a lane's CH genes.  Up to 4 genes per lane: one 64-bit
+// wave lane mask per gene (bit `lane` = that lane's cell), in scalar
+// registers, so the switch tests produce masks straight from v_cmp, a flip is
+// one s_xor_b64 and the gating of transcription one v_cndmask on the mask.
+// Wider tiles keep one bit per gene in a per-lane register instead: 2*CH
+// mask registers would not fit in the scalar file beside the step's other
+// uniforms, and their spills cost a v_readlane at every use.
+template <int CH>
+struct Promoters {
+    static constexpr bool kMasks = CH <= 4;
+    uint64_t mask[kMasks ? CH : 1];
+    uint32_t bits;
+    __device__ __forceinline__ void set(int k, bool o) {
+        if constexpr (kMasks) mask[k] = __builtin_amdgcn_ballot_w64(o);
+        else bits = (bits & ~(1u << k)) | ((uint32_t)o << k);
+    }
+    __device__ __forceinline__ bool get(int k) const {
+        if constexpr (kMasks) return __builtin_amdgcn_inverse_ballot_w64(mask[k]);
+        else return (bits >> k) & 1u;
+    }
+    __device__ __forceinline__ void flip(int k, uint64_t f) {
+        if constexpr (kMasks) mask[k] ^= f;
+        else bits ^= (uint32_t)__builtin_amdgcn_inverse_ballot_w64(f) << k;
+    }
+};
 
 // Occupancy and register slots for up to 4 genes per lane.  The kernel asks
 // for 7 waves per SIMD (72 VGPRs, 94 SGPRs), which holds two register slots
@@ -172,11 +197,8 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
     const int g0 = __builtin_amdgcn_readfirstlane(wave * CH);
 
     float m[CH], p[CH];
-    // Promoter states as wave lane masks: bit `lane` of on[k] is the state of
-    // gene g0+k in that lane's cell.  The switch tests then produce masks
-    // straight from v_cmp and combine them on the scalar unit, and the gating
-    // of transcription is one v_cndmask on the mask.
-    uint64_t on[CH];
+    Promoters<CH> on;                      // promoter state of gene g0+k in this lane's cell
+    on.bits = 0u;
     // Per-gene constants of the step loop kept in VGPRs (uniform values; VALU
     // operands at no cost, and no scalar registers or reloads across the
     // loop): the OFF->ON and ON->OFF thresholds in units of 2^14,
@@ -252,7 +274,7 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
         const int g = g0 + k;
         m[k] = 0.0f;
         p[k] = 0.0f;
-        on[k] = ~0ull;
+        on.set(k, true);
         if (g < G) {
             const uint32_t tf = telew[g * 8 + T_FLAGS];
             sw_mask |= (tf & 1u) << k;
@@ -271,7 +293,7 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
                     o = (uint64_t)uu[g & 3] < u64_at(telew, g * 8 + T_INIT_LO);
                 }
             }
-            on[k] = __builtin_amdgcn_ballot_w64(o);
+            on.set(k, o);
         }
     }
     if (pop) V = vol_pool[off_vol + cell];   // written by the population init kernel or a previous phase
@@ -345,9 +367,10 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
 #pragma unroll
                     for (int j = 0; j < 2 && k + j < CH; ++j) {
                         if (k + j < nv && ((swm >> (k + j)) & 1u))
-                            on[k + j] = cle_switch(on[k + j], j ? u.z : u.x, j ? u.w : u.y, thr_on[k + j],
-                                                   thr_off[k + j], (uint32_t)cell, (uint32_t)s, g + j, netc | kSwitch,
-                                                   sk0, sk1, blobs, boff + Bs[H_OFF_TELE] + (uint32_t)(g + j) * 8u);
+                            on.flip(k + j, cle_switch(on.get(k + j), j ? u.z : u.x, j ? u.w : u.y, thr_on[k + j],
+                                                      thr_off[k + j], (uint32_t)cell, (uint32_t)s, g + j,
+                                                      netc | kSwitch, sk0, sk1, blobs,
+                                                      boff + Bs[H_OFF_TELE] + (uint32_t)(g + j) * 8u));
                     }
                     if (inj) {
                         const int Gs = (int)Bs[H_G];
@@ -397,7 +420,7 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
                                 }
                             }
                             const float beta = beta_v[k + j];
-                            const bool o = __builtin_amdgcn_inverse_ballot_w64(on[k + j]);
+                            const bool o = on.get(k + j);
                             // telegraph gating of the transcription propensity
                             // (tauleap_binomial.jl:134-137); volume scaling (cle.jl:224-225)
                             const float drive = POP ? (reg + beta) * (float)V : reg + beta;
@@ -447,7 +470,7 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
             if (g < G) {
                 state_pool[off_state + (size_t)g * C + cell] = m[k];
                 state_pool[off_state + (size_t)(G + g) * C + cell] = p[k];
-                prom_pool[off_prom + (size_t)g * C + cell] = (uint8_t)__builtin_amdgcn_inverse_ballot_w64(on[k]);
+                prom_pool[off_prom + (size_t)g * C + cell] = (uint8_t)on.get(k);
             }
         }
         if (pop && wave == 0) vol_pool[off_vol + cell] = V;
