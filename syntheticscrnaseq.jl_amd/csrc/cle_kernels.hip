@@ -141,11 +141,12 @@ struct Promoters {
 constexpr int kEdgeSlots4 = 2;
 constexpr int kWavesPerSimd4 = 7;
 //
-// INJ and POP say whether a launch may hold networks with injected noise or a
-// population; the flags in each blob still decide per network.  Launches
-// without them (the common case) get a kernel without those paths, whose
-// pointers, offsets and flags would otherwise occupy scalar registers for the
-// whole horizon.
+// INJ, POP and TELE say whether a launch may hold networks with injected
+// noise, a population or telegraph promoters; the flags in each blob still
+// decide per network.  Launches without them get a kernel without those
+// paths, whose pointers, offsets and flags would otherwise occupy registers
+// for the whole horizon (a launch holding a telegraph network must use
+// TELE = true: without it every promoter stays ON).
 //
 // Scalar-register economy.  The step loop is long and every uniform value the
 // compiler keeps live across it takes an SGPR (a uniform boolean takes two, as
@@ -154,7 +155,7 @@ constexpr int kWavesPerSimd4 = 7;
 // opaque at the top of every step: header words are re-read through the
 // scalar cache and per-gene tests recomputed on the scalar unit instead of
 // being hoisted (booleans as lane masks) out of the loop.
-template <int CH, bool INJ, bool POP>
+template <int CH, bool INJ, bool POP, bool TELE>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(CH <= 4 ? kWavesPerSimd4 : 1)))
 cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ tiles,
                    float* __restrict__ state_pool, float* __restrict__ snap_pool,
@@ -182,7 +183,7 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
     const uint32_t flags = B[H_FLAGS];
     const int snap_every = (int)B[H_SNAP_EVERY];
     const int C = (int)B[H_CELLS];
-    const bool tele = (flags & F_TELE) != 0;
+    const bool tele = TELE && (flags & F_TELE) != 0;
     const bool pop = POP && (flags & F_POP) != 0;
     const uint32_t* __restrict__ telew = B + B[H_OFF_TELE];
     const uint32_t* __restrict__ initw = B + B[H_OFF_INIT];
@@ -359,14 +360,14 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
                     // injected-noise runs draw it for telegraph networks too.
                     const bool inj = INJ && (fl & F_INJECT);
                     U4 u = {0u, 0u, 0u, 0u};
-                    if (!inj || swm != 0u)
+                    if (!inj || (TELE && swm != 0u))
                         u = philox4x32_10((uint32_t)cell, (uint32_t)s, (uint32_t)(g >> 1), netc | kNormal, sk0, sk1);
                     // Telegraph switching of the two genes (tauleap_binomial.jl:110-121),
                     // ahead of their transcription below; the Hill values written
                     // above do not depend on promoter states.
 #pragma unroll
                     for (int j = 0; j < 2 && k + j < CH; ++j) {
-                        if (k + j < nv && ((swm >> (k + j)) & 1u))
+                        if (TELE && k + j < nv && ((swm >> (k + j)) & 1u))
                             on.flip(k + j, cle_switch(on.get(k + j), j ? u.z : u.x, j ? u.w : u.y, thr_on[k + j],
                                                       thr_off[k + j], (uint32_t)cell, (uint32_t)s, g + j,
                                                       netc | kSwitch, sk0, sk1, blobs,
@@ -696,31 +697,40 @@ hipError_t launch_pop_place(int n_tiles, const uint32_t* blobs, const Tile* tile
 size_t stepper_lds_bytes(int S, int nbuf) { return (size_t)nbuf * (2 * (size_t)S + 1) * 64 * sizeof(float); }
 
 template <int CH>
-static const void* stepper_fn(bool inj, bool pop) {
-    if (inj) return pop ? (const void*)cle_stepper_kernel<CH, true, true> : (const void*)cle_stepper_kernel<CH, true, false>;
-    return pop ? (const void*)cle_stepper_kernel<CH, false, true> : (const void*)cle_stepper_kernel<CH, false, false>;
+static const void* stepper_fn(bool inj, bool pop, bool tele) {
+    const void* fns[8] = {(const void*)cle_stepper_kernel<CH, false, false, false>,
+                          (const void*)cle_stepper_kernel<CH, true, false, false>,
+                          (const void*)cle_stepper_kernel<CH, false, true, false>,
+                          (const void*)cle_stepper_kernel<CH, true, true, false>,
+                          (const void*)cle_stepper_kernel<CH, false, false, true>,
+                          (const void*)cle_stepper_kernel<CH, true, false, true>,
+                          (const void*)cle_stepper_kernel<CH, false, true, true>,
+                          (const void*)cle_stepper_kernel<CH, true, true, true>};
+    return fns[(inj ? 1 : 0) | (pop ? 2 : 0) | (tele ? 4 : 0)];
 }
 
-static const void* stepper_fn(int CH, bool inj, bool pop) {
+static const void* stepper_fn(int CH, bool inj, bool pop, bool tele) {
     switch (CH) {
-        case 2: return stepper_fn<2>(inj, pop);
-        case 4: return stepper_fn<4>(inj, pop);
-        case 8: return stepper_fn<8>(inj, pop);
-        case 16: return stepper_fn<16>(inj, pop);
+        case 2: return stepper_fn<2>(inj, pop, tele);
+        case 4: return stepper_fn<4>(inj, pop, tele);
+        case 8: return stepper_fn<8>(inj, pop, tele);
+        case 16: return stepper_fn<16>(inj, pop, tele);
         default: return nullptr;
     }
 }
 
-// A launch with no injected-noise pool (no network of it injects noise) or no
-// volume pool (no population) runs the kernel variant without that path.
+// A launch with no injected-noise pool (no network of it injects noise), no
+// volume pool (no population) or tele = false (no telegraph network) runs the
+// kernel variant without that path.
 hipError_t launch_stepper(int CH, int NW, int n_tiles, const uint32_t* blobs, const Tile* tiles,
                           float* state_pool, float* snap_pool, uint8_t* prom_pool, double* vol_pool,
                           const float* inj_pool, int s_begin, int s_end, int nbuf,
-                          size_t lds_bytes, int* sched, int n_seg, long long wait_ticks, hipStream_t stream) {
+                          size_t lds_bytes, int* sched, int n_seg, long long wait_ticks, bool tele,
+                          hipStream_t stream) {
     if (n_tiles <= 0) return hipSuccess;
     if (sched == nullptr) n_seg = 1;
     if (n_seg < 1 || (long long)n_tiles * n_seg > INT32_MAX) return hipErrorInvalidValue;
-    const void* f = stepper_fn(CH, inj_pool != nullptr, vol_pool != nullptr);
+    const void* f = stepper_fn(CH, inj_pool != nullptr, vol_pool != nullptr, tele);
     if (!f) return hipErrorInvalidValue;
     void* args[] = {(void*)&blobs, (void*)&tiles, (void*)&state_pool, (void*)&snap_pool, (void*)&prom_pool,
                     (void*)&vol_pool, (void*)&inj_pool, (void*)&s_begin, (void*)&s_end, (void*)&nbuf,
@@ -736,7 +746,7 @@ hipError_t launch_sched_reset(int* sched, int n_words, hipStream_t stream) {
 }
 
 int stepper_blocks_per_cu(int CH, int NW, size_t lds_bytes) {
-    const void* f = stepper_fn(CH, false, false);
+    const void* f = stepper_fn(CH, false, false, true);
     if (!f) return 0;
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 64 * NW, lds_bytes) != hipSuccess) return 0;
@@ -748,9 +758,9 @@ hipError_t set_stepper_lds_limit(size_t bytes) {
     bytes = bytes > kStepperStaticLds ? bytes - kStepperStaticLds : bytes;
     hipError_t e = hipSuccess;
     for (int CH : {2, 4, 8, 16})
-        for (int v = 0; v < 4; ++v) {
-            hipError_t r = hipFuncSetAttribute(stepper_fn(CH, v & 1, v & 2), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                               (int)bytes);
+        for (int v = 0; v < 8; ++v) {
+            hipError_t r = hipFuncSetAttribute(stepper_fn(CH, v & 1, v & 2, v & 4),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
             if (r != hipSuccess) e = r;
         }
     return e;
