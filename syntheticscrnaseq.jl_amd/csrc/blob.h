@@ -94,7 +94,8 @@ enum BlobFlags : uint32_t {
     F_HILL2 = 2u,     // Hill coefficient is exactly 2
     F_GATES = 4u,     // AND / OR gates present
     F_INJECT = 8u,    // read normals from the injected-noise buffer
-    F_POP = 16u       // population dynamics (volume-scaled transcription)
+    F_POP = 16u,      // population dynamics (volume-scaled transcription)
+    F_NEG_INIT = 32u  // some initial count is negative (negative basal rates)
 };
 
 enum TeleWord : int {

@@ -41,12 +41,13 @@ int stepper_blocks_per_cu(int CH, int NW, size_t lds_bytes);
 // sched != nullptr: n_seg segments per tile, ordered by tickets; sched must be
 // reset (launch_sched_reset) before every such launch.  wait_ticks bounds a
 // segment's wait for its predecessor, in wall-clock ticks.  tele: some network
-// of the launch has telegraph promoters (selects the kernel with switching).
+// of the launch has telegraph promoters (selects the kernel with switching);
+// clamp: some network starts from negative counts (F_NEG_INIT).
 hipError_t launch_stepper(int CH, int NW, int n_tiles, const uint32_t* blobs, const Tile* tiles,
                           float* state_pool, float* snap_pool, uint8_t* prom_pool, double* vol_pool,
                           const float* inj_pool, int s_begin, int s_end, int nbuf,
                           size_t lds_bytes, int* sched, int n_seg, long long wait_ticks, bool tele,
-                          hipStream_t stream);
+                          bool clamp, hipStream_t stream);
 hipError_t launch_sched_reset(int* sched, int n_words, hipStream_t stream);
 // Binomial tau-leap stepper (tauleap_kernels.hip); same tiles and schedule.
 size_t tauleap_lds_bytes(int S, int nbuf, int NW, int CH);

@@ -148,6 +148,13 @@ constexpr int kWavesPerSimd4 = 7;
 // for the whole horizon (a launch holding a telegraph network must use
 // TELE = true: without it every promoter stays ON).
 //
+// CLAMP keeps the update's max(m, 0) and max(p, 0) and the protein
+// variance's max(., 0) (cle.jl:111, 118).  Every count is >= 0 at the start
+// of a step (each step ends with max(., 0); resumed, divided and placed states
+// come from such steps) unless the initial values are negative (negative
+// basal rates, F_NEG_INIT), and k_t, mu_p >= 0, so launches without such a
+// network drop them: 7 VALU instructions fewer per 4 genes, same results.
+//
 // Scalar-register economy.  The step loop is long and every uniform value the
 // compiler keeps live across it takes an SGPR (a uniform boolean takes two, as
 // a lane mask); past the budget they spill into VGPR lanes and each reload is
@@ -155,7 +162,7 @@ constexpr int kWavesPerSimd4 = 7;
 // opaque at the top of every step: header words are re-read through the
 // scalar cache and per-gene tests recomputed on the scalar unit instead of
 // being hoisted (booleans as lane masks) out of the loop.
-template <int CH, bool INJ, bool POP, bool TELE>
+template <int CH, bool INJ, bool POP, bool TELE, bool CLAMP>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(CH <= 4 ? kWavesPerSimd4 : 1)))
 cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ tiles,
                    float* __restrict__ state_pool, float* __restrict__ snap_pool,
@@ -395,7 +402,8 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
                         if (k + j < nv) {
                             // Additive Hill regulation (cle.jl:84-86): the register slots,
                             // then any further (LDS row, weight) pairs of the blob.
-                            float reg = 0.0f;
+                            // basal rate first: the sum is the whole transcription drive
+                            float reg = beta_v[k + j];
 #pragma unroll
                             for (int i = 0; i < EK; ++i)
                                 reg = fmaf(ew[k + j][i], *(const float*)((const char*)lds + (eaddr[k + j][i] + hbase)),
@@ -420,21 +428,27 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
                                     q += 2 + ns;
                                 }
                             }
-                            const float beta = beta_v[k + j];
                             const bool o = on.get(k + j);
                             // telegraph gating of the transcription propensity
                             // (tauleap_binomial.jl:134-137); volume scaling (cle.jl:224-225)
-                            const float drive = POP ? (reg + beta) * (float)V : reg + beta;
+                            const float drive = POP ? reg * (float)V : reg;
                             const float tx = o ? drive : 0.0f;
                             // mRNA (cle.jl:110-113)
                             const float mm = m[k + j];
-                            const float var_m = fmaxf(fmaf(mu_m, fmaxf(mm, 0.0f), tx), 0.0f);
+                            const float var_m = fmaxf(fmaf(mu_m, CLAMP ? fmaxf(mm, 0.0f) : mm, tx), 0.0f);
                             float mn = fmaf(fmaf(-mu_m, mm, tx), dt, mm);
                             mn = fmaxf(fmaf(c[j], __builtin_amdgcn_sqrtf(r2[j] * var_m), mn), 0.0f);
                             // protein with the updated mRNA (cle.jl:116-120)
                             const float pp = p[k + j];
-                            const float var_p = fmaxf(fmaf(k_t, mn, mu_p * fmaxf(pp, 0.0f)), 0.0f);
-                            float pn = fmaf(fmaf(k_t, mn, -mu_p * pp), dt, pp);
+                            float var_p, pn;
+                            if (CLAMP) {
+                                var_p = fmaxf(fmaf(k_t, mn, mu_p * fmaxf(pp, 0.0f)), 0.0f);
+                                pn = fmaf(fmaf(k_t, mn, -mu_p * pp), dt, pp);
+                            } else {
+                                const float dp = mu_p * pp;        // protein decay propensity
+                                var_p = fmaf(k_t, mn, dp);
+                                pn = fmaf(fmaf(k_t, mn, -dp), dt, pp);
+                            }
                             pn = fmaxf(fmaf(sn[j], __builtin_amdgcn_sqrtf(r2[j] * var_p), pn), 0.0f);
                             m[k + j] = mn;
                             p[k + j] = pn;
@@ -697,40 +711,42 @@ hipError_t launch_pop_place(int n_tiles, const uint32_t* blobs, const Tile* tile
 size_t stepper_lds_bytes(int S, int nbuf) { return (size_t)nbuf * (2 * (size_t)S + 1) * 64 * sizeof(float); }
 
 template <int CH>
-static const void* stepper_fn(bool inj, bool pop, bool tele) {
-    const void* fns[8] = {(const void*)cle_stepper_kernel<CH, false, false, false>,
-                          (const void*)cle_stepper_kernel<CH, true, false, false>,
-                          (const void*)cle_stepper_kernel<CH, false, true, false>,
-                          (const void*)cle_stepper_kernel<CH, true, true, false>,
-                          (const void*)cle_stepper_kernel<CH, false, false, true>,
-                          (const void*)cle_stepper_kernel<CH, true, false, true>,
-                          (const void*)cle_stepper_kernel<CH, false, true, true>,
-                          (const void*)cle_stepper_kernel<CH, true, true, true>};
-    return fns[(inj ? 1 : 0) | (pop ? 2 : 0) | (tele ? 4 : 0)];
+static const void* stepper_fn(bool inj, bool pop, bool tele, bool clamp) {
+#define SCRNA_STEPPER_VARIANTS(CL)                                                                       \
+    (const void*)cle_stepper_kernel<CH, false, false, false, CL>,                                        \
+        (const void*)cle_stepper_kernel<CH, true, false, false, CL>,                                     \
+        (const void*)cle_stepper_kernel<CH, false, true, false, CL>,                                     \
+        (const void*)cle_stepper_kernel<CH, true, true, false, CL>,                                      \
+        (const void*)cle_stepper_kernel<CH, false, false, true, CL>,                                     \
+        (const void*)cle_stepper_kernel<CH, true, false, true, CL>,                                      \
+        (const void*)cle_stepper_kernel<CH, false, true, true, CL>,                                      \
+        (const void*)cle_stepper_kernel<CH, true, true, true, CL>
+    const void* fns[16] = {SCRNA_STEPPER_VARIANTS(false), SCRNA_STEPPER_VARIANTS(true)};
+#undef SCRNA_STEPPER_VARIANTS
+    return fns[(inj ? 1 : 0) | (pop ? 2 : 0) | (tele ? 4 : 0) | (clamp ? 8 : 0)];
 }
-
-static const void* stepper_fn(int CH, bool inj, bool pop, bool tele) {
+static const void* stepper_fn(int CH, bool inj, bool pop, bool tele, bool clamp) {
     switch (CH) {
-        case 2: return stepper_fn<2>(inj, pop, tele);
-        case 4: return stepper_fn<4>(inj, pop, tele);
-        case 8: return stepper_fn<8>(inj, pop, tele);
-        case 16: return stepper_fn<16>(inj, pop, tele);
+        case 2: return stepper_fn<2>(inj, pop, tele, clamp);
+        case 4: return stepper_fn<4>(inj, pop, tele, clamp);
+        case 8: return stepper_fn<8>(inj, pop, tele, clamp);
+        case 16: return stepper_fn<16>(inj, pop, tele, clamp);
         default: return nullptr;
     }
 }
 
 // A launch with no injected-noise pool (no network of it injects noise), no
-// volume pool (no population) or tele = false (no telegraph network) runs the
-// kernel variant without that path.
+// volume pool (no population), tele = false (no telegraph network) or clamp =
+// false (no negative initial value) runs the kernel variant without that path.
 hipError_t launch_stepper(int CH, int NW, int n_tiles, const uint32_t* blobs, const Tile* tiles,
                           float* state_pool, float* snap_pool, uint8_t* prom_pool, double* vol_pool,
                           const float* inj_pool, int s_begin, int s_end, int nbuf,
                           size_t lds_bytes, int* sched, int n_seg, long long wait_ticks, bool tele,
-                          hipStream_t stream) {
+                          bool clamp, hipStream_t stream) {
     if (n_tiles <= 0) return hipSuccess;
     if (sched == nullptr) n_seg = 1;
     if (n_seg < 1 || (long long)n_tiles * n_seg > INT32_MAX) return hipErrorInvalidValue;
-    const void* f = stepper_fn(CH, inj_pool != nullptr, vol_pool != nullptr, tele);
+    const void* f = stepper_fn(CH, inj_pool != nullptr, vol_pool != nullptr, tele, clamp);
     if (!f) return hipErrorInvalidValue;
     void* args[] = {(void*)&blobs, (void*)&tiles, (void*)&state_pool, (void*)&snap_pool, (void*)&prom_pool,
                     (void*)&vol_pool, (void*)&inj_pool, (void*)&s_begin, (void*)&s_end, (void*)&nbuf,
@@ -746,7 +762,7 @@ hipError_t launch_sched_reset(int* sched, int n_words, hipStream_t stream) {
 }
 
 int stepper_blocks_per_cu(int CH, int NW, size_t lds_bytes) {
-    const void* f = stepper_fn(CH, false, false, true);
+    const void* f = stepper_fn(CH, false, false, true, false);
     if (!f) return 0;
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 64 * NW, lds_bytes) != hipSuccess) return 0;
@@ -758,8 +774,8 @@ hipError_t set_stepper_lds_limit(size_t bytes) {
     bytes = bytes > kStepperStaticLds ? bytes - kStepperStaticLds : bytes;
     hipError_t e = hipSuccess;
     for (int CH : {2, 4, 8, 16})
-        for (int v = 0; v < 8; ++v) {
-            hipError_t r = hipFuncSetAttribute(stepper_fn(CH, v & 1, v & 2, v & 4),
+        for (int v = 0; v < 16; ++v) {
+            hipError_t r = hipFuncSetAttribute(stepper_fn(CH, v & 1, v & 2, v & 4, v & 8),
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
             if (r != hipSuccess) e = r;
         }

@@ -438,6 +438,7 @@ int compile_network(const scrna_network* net, const scrna_kinetics* kin, const s
         const double pss = kin->k_t * mss / (kin->mu_p + kin->dilution);   // kinetics.jl:68
         const float m0 = run->init_mode == 0 ? (float)std::ceil(mss) : 0.0f;
         const float p0 = run->init_mode == 0 ? (float)std::ceil(pss) : 0.0f;
+        if (m0 < 0.0f || p0 < 0.0f) b[H_FLAGS] |= F_NEG_INIT;
         b.push_back(f2w(m0));
         b.push_back(f2w(p0));
     }
@@ -635,6 +636,7 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
     std::map<GKey, std::pair<int, int>> group_SG;   // max S, max Gp
     std::map<GKey, int64_t> group_steps;            // longest horizon
     std::map<GKey, bool> group_tele;                // some network has telegraph promoters
+    std::map<GKey, bool> group_clamp;               // some network starts from negative counts
     size_t w_elems = 0;
     for (int i = 0; i < n; ++i) {
         CompiledNet& cn = P->nets[i];
@@ -662,6 +664,7 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
                             : GKey{cn.algorithm == SCRNA_ALG_BINOMIAL_TAU_LEAP ? 2 : 0, cn.CH, cn.NW, 0, 0};
         auto& v = groups[key];
         group_tele[key] = group_tele[key] || tele;
+        group_clamp[key] = group_clamp[key] || (cn.blob[H_FLAGS] & F_NEG_INIT) != 0;
         auto& sg = group_SG[key];
         sg.first = std::max(sg.first, cn.S);
         sg.second = std::max(sg.second, cn.Gp);
@@ -698,6 +701,7 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
             const bool tl = lg.kind == 2;
             lg.CH = kv.first.a;
             lg.tele = group_tele[kv.first];
+            lg.clamp = group_clamp[kv.first];
             const size_t two = tl ? tauleap_lds_bytes(S, 2, lg.NW, lg.CH) : stepper_lds_bytes(S, 2);
             lg.nbuf = two <= ctx->lds_limit_soft ? 2 : 1;
             lg.lds = tl ? tauleap_lds_bytes(S, lg.nbuf, lg.NW, lg.CH) : stepper_lds_bytes(S, lg.nbuf);
@@ -935,7 +939,7 @@ int launch_steps(scrna_plan* P, hipStream_t st, int s_begin, int s_end, bool seg
             if (lg.kind == 0)
                 HIP_TRY(launch_stepper(lg.CH, lg.NW, lg.n_tiles, P->d_blobs, P->d_tiles + lg.tile0, P->d_state,
                                        P->d_snap, P->d_prom, P->d_vol, P->d_inj, s_begin, s_end, lg.nbuf, lg.lds,
-                                       sched, nseg, P->wait_ticks, lg.tele, gs));
+                                       sched, nseg, P->wait_ticks, lg.tele, lg.clamp, gs));
             else
                 HIP_TRY(launch_tauleap(lg.CH, lg.NW, lg.n_tiles, P->d_blobs, P->d_tiles + lg.tile0, P->d_state,
                                        P->d_snap, P->d_prom, P->d_vol, s_begin, s_end, lg.nbuf, lg.lds, sched,

@@ -73,6 +73,7 @@ struct LaunchGroup {
     int CH = 2, NW = 1, tile0 = 0, n_tiles = 0, nbuf = 1;
     int mode = 0;                    // dense precision mode
     bool tele = false, inject = false;
+    bool clamp = false;              // FP32 stepper: some network starts from negative counts
     size_t lds = 0;
     int nseg = 1;                    // FP32 stepper: segments per tile on whole-horizon launches
     size_t sched_off = 0;            // its schedule block in scrna_plan::d_sched

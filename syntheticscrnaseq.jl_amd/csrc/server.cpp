@@ -305,6 +305,7 @@ struct RoundGroup {
     int CH = 0, NW = 0, Smax = 1;
     bool multi = false;                    // requests advancing several chunks this round
     bool tele = false;                     // some request has telegraph promoters
+    bool clamp = false;                    // some request starts from negative counts
     int64_t min_steps = INT64_MAX;         // fewest steps any of its tiles takes
     std::vector<Tile> tiles;
 };
@@ -327,6 +328,7 @@ hipError_t run_round(scrna_server* S, const std::vector<Req*>& run) {
         g.multi = k > 1;
         g.Smax = std::max(g.Smax, r->cn.S);
         g.tele = g.tele || (r->cn.blob[H_FLAGS] & F_TELE) != 0;
+        g.clamp = g.clamp || (r->cn.blob[H_FLAGS] & F_NEG_INIT) != 0;
         g.min_steps = std::min<int64_t>(g.min_steps, s1 - s0);
         for (int64_t c0 = 0; c0 < r->cn.cells; c0 += 64) {
             Tile t{};
@@ -387,7 +389,7 @@ hipError_t run_round(scrna_server* S, const std::vector<Req*>& run) {
         if (e == hipSuccess)
             e = launch_stepper(g.CH, g.NW, (int)g.tiles.size(), S->blob_arena, S->d_tiles + tile0[gi],
                                (float*)S->arena, nullptr, (uint8_t*)S->arena, nullptr, nullptr, 0, INT32_MAX, nbuf,
-                               stepper_lds_bytes(g.Smax, nbuf), sched, nseg[gi], S->wait_ticks, g.tele, st);
+                               stepper_lds_bytes(g.Smax, nbuf), sched, nseg[gi], S->wait_ticks, g.tele, g.clamp, st);
         ++gi;
     }
     for (int i = 0; i < S->kStreams - 1; ++i) {

@@ -7,7 +7,10 @@ FP64 oracle on identical injected noise (tolerances as in test_gpu_parity:
   * genes with more in-edges than the stepper keeps in registers (the rest
     stream from the network descriptor), for every genes-per-lane tile;
   * AND / OR gates with three sources and targets with two gates (the
-    dense kernel keeps only one two-source gate per gene in registers).
+    dense kernel keeps only one two-source gate per gene in registers);
+  * negative initial counts (negative basal rates): the first step keeps the
+    update's max(m, 0), max(p, 0) clamps, which launches without such a
+    network leave out.
 """
 import numpy as np
 import pytest
@@ -86,3 +89,22 @@ def test_dense_gate_layouts(ctx, three_source):
     err = run_both(net, 50, 0.005, 0.5, 9, precision="bf16x3", ctx=ctx)
     print(f"max rel err {err:.3e}")
     assert err <= 2e-4
+
+
+def test_negative_initial_counts_keep_the_clamps(ctx):
+    # ceil(beta / mu_m) < 0 for genes 1 and 3: their first step reads max(m, 0) = 0.
+    A = np.array([[0, 1.5, 0, 0, 0], [0, 0, -2.0, 0, 0], [0, 0, 0, 1.0, 0], [0, 0, 0, 0, 2.5],
+                  [-1.0, 0, 0, 0, 0]])
+    net = sc.GeneNetwork(5, np.array([2.0, -2.5, 1.0, -0.7, 3.0]), A, [], [],
+                         k_on=[0.5, np.inf, 1.0, 0.3, np.inf], k_off=[2.0, np.inf, 1.5, 4.0, np.inf])
+    assert run_both(net, 130, 0.01, 1.0, 41, ctx=ctx) <= 1e-5
+    # in a batch the clamped kernel serves the whole launch group (free Philox streams, net_id = index)
+    rng = np.random.default_rng(5)
+    other = sc.samplers.mixed_network(5, rng, n_tf=(1, 2), targets_per_tf=(1, 3), n_and=(0, 1), n_or=(0, 1),
+                                      telegraph=True)
+    nets, cells = [net, other], [130, 70]
+    out = sc.simulate_gpu_batch(nets, sc.CLE(0.01), KIN, cell_num=cells, T=1.0, readout="both", seed=9,
+                                full=True, ctx=ctx)
+    for i, (n_, c) in enumerate(zip(nets, cells)):
+        ref = orc.cle(n_, KIN, dt=0.01, T=1.0, cell_num=c, seed=9, net_id=i)
+        assert relerr(out[i].y, np.concatenate([ref["m"].T, ref["p"].T], axis=1)) <= 2e-4
