@@ -72,7 +72,7 @@ hipError_t launch_pop_place(int n_tiles, const uint32_t* blobs, const Tile* tile
 size_t dense_lds_bytes(int Gp, int S, int mode);
 hipError_t set_dense_lds_limit(size_t bytes);
 int dense_nb(int mode);
-hipError_t launch_dense(int mode, bool tele, bool inject, int NW, int n_tiles, const uint32_t* blobs,
+hipError_t launch_dense(int mode, bool tele, bool inject, bool clamp, int NW, int n_tiles, const uint32_t* blobs,
                         const Tile* tiles, const void* wpool, float* state_pool, float* snap_pool, uint8_t* prom_pool,
                         const float* inj_pool, int s_begin, int s_end, size_t lds_bytes,
                         hipStream_t stream);

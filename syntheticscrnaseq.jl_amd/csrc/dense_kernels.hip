@@ -54,7 +54,11 @@ __device__ __forceinline__ uint32_t dpp_xor1(uint32_t x) {
     return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
 }
 
-template <int MODE, int NB, bool TELE, bool INJECT>
+// CLAMP keeps max(m, 0), max(p, 0) and the protein variance's max(., 0) of
+// the update, which only a network starting from negative counts
+// (F_NEG_INIT) needs: as in the FP32 stepper, counts are never negative at
+// the start of a later step.
+template <int MODE, int NB, bool TELE, bool INJECT, bool CLAMP>
 __global__ void __launch_bounds__(1024 / NB)
 cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ tiles,
                  const bf16x8* __restrict__ wpool, float* __restrict__ state_pool, float* __restrict__ snap_pool,
@@ -118,7 +122,7 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
     float gw[NB];
     int ga[NB], gb[NB];
     bool gor[NB], gslow[NB], wave_gate[NB], wave_slow[NB];
-    float beta[NB], bias[NB];
+    float bias[NB];                        // basal rate + rowsum(A_neg) bias: the drive's constant part
     uint32_t tflags[NB];
     uint32_t thr_on[NB], thr_off[NB];     // floor(threshold / 2^14), see cle_switch
     float m[NB][4], p[NB][4];
@@ -144,8 +148,7 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
         // Padding genes (g >= G) get beta = bias = 0, zero weight rows and
         // columns, an OFF promoter and zero state, so every step leaves them at
         // exactly 0 and the update needs no mask.
-        beta[b] = gl[b] ? d_as_f(B[B[H_OFF_BETA] + gc]) : 0.0f;
-        bias[b] = gl[b] ? d_as_f(B[B[H_OFF_BIAS] + gc]) : 0.0f;
+        bias[b] = gl[b] ? d_as_f(B[B[H_OFF_BIAS] + gc]) + d_as_f(B[B[H_OFF_BETA] + gc]) : 0.0f;
         slot[b] = gl[b] ? ((const int32_t*)(B + B[H_OFF_SLOT]))[g[b]] : -1;
         tflags[b] = gl[b] ? telew[g[b] * 8 + T_FLAGS] : 0u;
         thr_on[b] = telew[gc * 8 + T_ON_HI] != 0u ? 1u << 18 : telew[gc * 8 + T_ON_LO] >> 14;
@@ -306,14 +309,21 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
                 on ^= (uint32_t)flip << bit;
             }
             const bool o = (on >> (4 * b + v)) & 1u;
-            const float tx = o ? reg + beta[b] : 0.0f;
+            const float tx = o ? reg : 0.0f;
             const float mm = m[b][v];
-            const float var_m = fmaxf(fmaf(mu_m, fmaxf(mm, 0.0f), tx), 0.0f);
+            const float var_m = fmaxf(fmaf(mu_m, CLAMP ? fmaxf(mm, 0.0f) : mm, tx), 0.0f);
             float mn = fmaf(fmaf(-mu_m, mm, tx), dt, mm);
             mn = fmaxf(fmaf(c, __builtin_amdgcn_sqrtf(r2 * var_m), mn), 0.0f);
             const float pp = p[b][v];
-            const float var_p = fmaxf(fmaf(k_t, mn, mu_p * fmaxf(pp, 0.0f)), 0.0f);
-            float pn = fmaf(fmaf(k_t, mn, -mu_p * pp), dt, pp);
+            float var_p, pn;
+            if (CLAMP) {
+                var_p = fmaxf(fmaf(k_t, mn, mu_p * fmaxf(pp, 0.0f)), 0.0f);
+                pn = fmaf(fmaf(k_t, mn, -mu_p * pp), dt, pp);
+            } else {
+                const float dp = mu_p * pp;
+                var_p = fmaf(k_t, mn, dp);
+                pn = fmaf(fmaf(k_t, mn, -dp), dt, pp);
+            }
             pn = fmaxf(fmaf(sn, __builtin_amdgcn_sqrtf(r2 * var_p), pn), 0.0f);
             m[b][v] = mn;
             p[b][v] = pn;
@@ -399,14 +409,23 @@ size_t dense_lds_bytes(int Gp, int S, int mode) {
 }
 
 template <int MODE, int NB>
+static const void* dense_fn(bool tele, bool inject, bool clamp) {
+#define SCRNA_DENSE_VARIANTS(CL)                                                                         \
+    (const void*)cle_dense_kernel<MODE, NB, false, false, CL>,                                          \
+        (const void*)cle_dense_kernel<MODE, NB, false, true, CL>,                                       \
+        (const void*)cle_dense_kernel<MODE, NB, true, false, CL>,                                       \
+        (const void*)cle_dense_kernel<MODE, NB, true, true, CL>
+    const void* fns[8] = {SCRNA_DENSE_VARIANTS(false), SCRNA_DENSE_VARIANTS(true)};
+#undef SCRNA_DENSE_VARIANTS
+    return fns[(inject ? 1 : 0) | (tele ? 2 : 0) | (clamp ? 4 : 0)];
+}
+
+template <int MODE, int NB>
 static hipError_t dense_attr(size_t bytes) {
-    const void* fns[] = {(const void*)cle_dense_kernel<MODE, NB, false, false>,
-                         (const void*)cle_dense_kernel<MODE, NB, false, true>,
-                         (const void*)cle_dense_kernel<MODE, NB, true, false>,
-                         (const void*)cle_dense_kernel<MODE, NB, true, true>};
     hipError_t e = hipSuccess;
-    for (const void* f : fns) {
-        hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    for (int v = 0; v < 8; ++v) {
+        hipError_t r = hipFuncSetAttribute(dense_fn<MODE, NB>(v & 2, v & 1, v & 4),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
         if (r != hipSuccess) e = r;
     }
     return e;
@@ -418,39 +437,19 @@ hipError_t set_dense_lds_limit(size_t bytes) {
     return e1 != hipSuccess ? e1 : e2;
 }
 
-template <int MODE, int NB>
-static void dense_go(bool tele, bool inject, dim3 grid, dim3 block, size_t lds, hipStream_t st, const uint32_t* blobs,
-                     const Tile* tiles, const bf16x8* wpool, float* state, float* snap, uint8_t* prom,
-                     const float* inj, int s_begin, int s_end) {
-    if (tele && inject)
-        hipLaunchKernelGGL((cle_dense_kernel<MODE, NB, true, true>), grid, block, lds, st, blobs, tiles, wpool, state,
-                           snap, prom, inj, s_begin, s_end);
-    else if (tele)
-        hipLaunchKernelGGL((cle_dense_kernel<MODE, NB, true, false>), grid, block, lds, st, blobs, tiles, wpool, state,
-                           snap, prom, inj, s_begin, s_end);
-    else if (inject)
-        hipLaunchKernelGGL((cle_dense_kernel<MODE, NB, false, true>), grid, block, lds, st, blobs, tiles, wpool, state,
-                           snap, prom, inj, s_begin, s_end);
-    else
-        hipLaunchKernelGGL((cle_dense_kernel<MODE, NB, false, false>), grid, block, lds, st, blobs, tiles, wpool,
-                           state, snap, prom, inj, s_begin, s_end);
-}
-
-hipError_t launch_dense(int mode, bool tele, bool inject, int NW, int n_tiles, const uint32_t* blobs,
+hipError_t launch_dense(int mode, bool tele, bool inject, bool clamp, int NW, int n_tiles, const uint32_t* blobs,
                         const Tile* tiles, const void* wpool, float* state_pool, float* snap_pool, uint8_t* prom_pool,
                         const float* inj_pool, int s_begin, int s_end, size_t lds_bytes,
                         hipStream_t stream) {
     if (n_tiles <= 0) return hipSuccess;
-    const dim3 grid(n_tiles), block(64 * NW);
-    if (mode == 1)
-        dense_go<1, 1>(tele, inject, grid, block, lds_bytes, stream, blobs, tiles, (const bf16x8*)wpool, state_pool,
-                       snap_pool, prom_pool, inj_pool, s_begin, s_end);
-    else if (mode == 2)
-        dense_go<2, 2>(tele, inject, grid, block, lds_bytes, stream, blobs, tiles, (const bf16x8*)wpool, state_pool,
-                       snap_pool, prom_pool, inj_pool, s_begin, s_end);
-    else
-        return hipErrorInvalidValue;
-    return hipGetLastError();
+    const void* f = mode == 1 ? dense_fn<1, 1>(tele, inject, clamp)
+                  : mode == 2 ? dense_fn<2, 2>(tele, inject, clamp)
+                              : nullptr;
+    if (!f) return hipErrorInvalidValue;
+    const bf16x8* w = (const bf16x8*)wpool;
+    void* args[] = {(void*)&blobs, (void*)&tiles, (void*)&w, (void*)&state_pool, (void*)&snap_pool,
+                    (void*)&prom_pool, (void*)&inj_pool, (void*)&s_begin, (void*)&s_end};
+    return hipLaunchKernel(f, dim3(n_tiles), dim3(64 * NW), args, lds_bytes, stream);
 }
 
 }  // namespace scrna

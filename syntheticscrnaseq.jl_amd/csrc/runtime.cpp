@@ -719,6 +719,7 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
         } else {
             lg.mode = kv.first.a;
             lg.tele = kv.first.tele != 0;
+            lg.clamp = group_clamp[kv.first];
             lg.inject = kv.first.inj != 0;
             lg.nbuf = 2;
             lg.lds = dense_lds_bytes(group_SG[kv.first].second, S, lg.mode);
@@ -945,7 +946,7 @@ int launch_steps(scrna_plan* P, hipStream_t st, int s_begin, int s_end, bool seg
                                        P->d_snap, P->d_prom, P->d_vol, s_begin, s_end, lg.nbuf, lg.lds, sched,
                                        nseg, P->wait_ticks, gs));
         } else
-            HIP_TRY(launch_dense(lg.mode, lg.tele, lg.inject, lg.NW, lg.n_tiles, P->d_blobs, P->d_tiles + lg.tile0,
+            HIP_TRY(launch_dense(lg.mode, lg.tele, lg.inject, lg.clamp, lg.NW, lg.n_tiles, P->d_blobs, P->d_tiles + lg.tile0,
                                  P->d_wpool, P->d_state, P->d_snap, P->d_prom, P->d_inj, s_begin, s_end, lg.lds,
                                  gs));
     }
