@@ -59,11 +59,12 @@ def valu_census(net, dense: bool = False) -> dict:
     Operation counts (the fewest per-lane instructions the algorithm needs on
     gfx950, so that the fraction of peak is a speed-of-light fraction):
       update E = 36 ops (cle.jl:79-120, every +,-,*,/,max,sqrt,pow = 1), of
-        which 2 sqrt and 1 reciprocal are transcendental and 33 FP32; the FP32
-        stepper needs 5 fewer (28 FP32): counts are never negative at the
+        which 2 sqrt and 1 reciprocal are transcendental and 33 FP32; the
+        steppers need 5 fewer (28 FP32): counts are never negative at the
         start of a step, so max(m, 0), max(p, 0) and the protein variance's
         max(., 0) are no-ops, the decay propensity mu_p*p serves drift and
-        variance, and the basal rate is the regulation sum's first addend;
+        variance, and the basal rate is the regulation sum's first addend
+        (the dense kernel's bias);
       regulation R = 2 * additive in-edges per gene (0 on the MFMA path, whose
         contraction is MFMA work) + 3 * gate source incidences per gene, FP32;
       Box-Muller for the pair of normals: log, sin, cos + 5 FP32 (its radius
@@ -85,7 +86,7 @@ def valu_census(net, dense: bool = False) -> dict:
     inc = sum(len(e.sources) for e in list(net.cooperative) + list(net.redundant)) / G
     tele = bool(np.any(np.isfinite(net.k_on)))
     calls = 0.5
-    counts = {"fp32": (33 if dense else 28) + 2 * d_add + 3 * inc + 5, "trans": 3 + 3,
+    counts = {"fp32": 28 + 2 * d_add + 3 * inc + 5, "trans": 3 + 3,
               "int": 20 * calls + (6 if tele else 0), "mul32": 20 * calls}
     fma_eq = (counts["fp32"] * COST_FP32 + counts["trans"] * COST_TRANS + counts["int"] * COST_INT
               + counts["mul32"] * COST_MUL32)
