@@ -85,7 +85,9 @@ int time_shape(scrna_ctx* ctx, SynthNet& s, int ch, int64_t cells, int steps, do
     int rc = plan_build(ctx, 1, &s.net, &kin, &run, nullptr, nullptr, ch, &P);
     if (rc) return rc;
     double best = 1e30;
-    for (int rep = 0; rep < 3 && rc == SCRNA_OK; ++rep) {
+    // The fastest of four timed launches after a warm-up one: shapes whose
+    // times differ by a few percent are then told apart run after run.
+    for (int rep = 0; rep < 5 && rc == SCRNA_OK; ++rep) {
         rc = plan_launch(P, ctx->stream);
         if (rc) break;
         if (hipEventSynchronize(P->ev_stop) != hipSuccess) { rc = SCRNA_ERR_DEVICE; break; }
@@ -188,7 +190,7 @@ int scrna_autotune(scrna_ctx* ctx, int32_t max_G, const char* cache_path, int32_
         for (int ch : {2, 4, 8, 16}) {
             if ((Gb + ch - 1) / ch > 16) continue;
             double ms = 0;
-            int rc = time_shape(ctx, s, ch, cells, 64, &ms);
+            int rc = time_shape(ctx, s, ch, cells, 128, &ms);
             if (rc) return rc;
             if (ms < best) { best = ms; best_ch = ch; }
         }
