@@ -324,7 +324,8 @@ def run_batch(a, sc) -> dict:
     # Tile shapes (genes per lane per G bucket) from the autotuner, timed on this device.
     t0 = time.perf_counter()
     tune_path = os.path.join(tempfile.mkdtemp(), "tiles.json")
-    sc.default_context().autotune(max_G=max(x.G for x in nets), cache_path=tune_path)
+    sc.default_context().autotune(max_G=max(x.G for x in nets), cache_path=tune_path,
+                                  phase_steps=pop.div_check_interval)
     t_tune = time.perf_counter() - t0
     tile_table = json.load(open(tune_path))["table"]
     pops = [sc.PopulationConfig(cell_num=c, growth_rate=pop.growth_rate, V_div=pop.V_div, V_init=pop.V_init,

@@ -218,6 +218,11 @@ int scrna_plan_destroy(scrna_plan* plan);
  * otherwise measured and written there.  No reference counterpart (the
  * reference GPU path has no tunable tiles). */
 int scrna_autotune(scrna_ctx* ctx, int32_t max_G, const char* cache_path, int32_t* n_buckets);
+/* The same, timing each shape the way population plans run it: launches of
+ * phase_steps steps (their div_check_interval) with the division kernels
+ * between them; phase_steps = 0 is scrna_autotune.  No reference counterpart. */
+int scrna_autotune_phases(scrna_ctx* ctx, int32_t max_G, int32_t phase_steps, const char* cache_path,
+                          int32_t* n_buckets);
 /* Pin the chunk for networks with G in [G_lo, G_hi]; genes_per_lane 0 clears. */
 int scrna_ctx_set_tile(scrna_ctx* ctx, int32_t G_lo, int32_t G_hi, int32_t genes_per_lane);
 

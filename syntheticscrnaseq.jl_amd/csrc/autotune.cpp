@@ -73,7 +73,9 @@ void synth(int G, SynthNet& s) {
     s.net.k_off = s.koff.data();
 }
 
-int time_shape(scrna_ctx* ctx, SynthNet& s, int ch, int64_t cells, int steps, double* ms) {
+// phase > 0: the plan runs population dynamics with a division check every
+// `phase` steps, so the stepper is launched `phase` steps at a time.
+int time_shape(scrna_ctx* ctx, SynthNet& s, int ch, int64_t cells, int steps, int phase, double* ms) {
     scrna_kinetics kin{5.0, 10.0, 2.0, 1.0, 0.2, 0.0};
     scrna_run run{};
     run.dt = 0.01;
@@ -81,8 +83,19 @@ int time_shape(scrna_ctx* ctx, SynthNet& s, int ch, int64_t cells, int steps, do
     run.cell_num = cells;
     run.readout = SCRNA_READOUT_PROTEIN;
     run.seed = 12345;
+    scrna_population pop{};
+    if (phase > 0) {
+        pop.enabled = 1;
+        pop.growth_rate = 0.05;
+        pop.V_div = 2.0;
+        pop.V_init_lo = 0.8;
+        pop.V_init_hi = 1.2;
+        pop.div_check_interval = phase;
+        pop.partition = 1;
+        pop.replace_mode = 0;
+    }
     scrna_plan* P = nullptr;
-    int rc = plan_build(ctx, 1, &s.net, &kin, &run, nullptr, nullptr, ch, &P);
+    int rc = plan_build(ctx, 1, &s.net, &kin, &run, phase > 0 ? &pop : nullptr, nullptr, ch, &P);
     if (rc) return rc;
     double best = 1e30;
     // The fastest of four timed launches after a warm-up one: shapes whose
@@ -164,12 +177,20 @@ int scrna_ctx_set_tile(scrna_ctx* ctx, int32_t G_lo, int32_t G_hi, int32_t ch) {
 }
 
 int scrna_autotune(scrna_ctx* ctx, int32_t max_G, const char* cache_path, int32_t* n_buckets) {
+    return scrna_autotune_phases(ctx, max_G, 0, cache_path, n_buckets);
+}
+
+int scrna_autotune_phases(scrna_ctx* ctx, int32_t max_G, int32_t phase_steps, const char* cache_path,
+                          int32_t* n_buckets) {
     if (!ctx) return set_error(SCRNA_ERR_INVALID, "null context");
     if (max_G < 1 || max_G > kMaxGenes) return set_error(SCRNA_ERR_INVALID, "max_G must be in [1, %d]", kMaxGenes);
+    if (phase_steps < 0) return set_error(SCRNA_ERR_INVALID, "phase_steps must be >= 0");
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess)
         return set_error(SCRNA_ERR_DEVICE, "hipGetDeviceProperties failed");
-    const std::string arch = std::string(prop.gcnArchName).substr(0, 6) + "/" + std::to_string(ctx->num_cu);
+    // The cache key holds the launch length the table was tuned for.
+    const std::string arch = std::string(prop.gcnArchName).substr(0, 6) + "/" + std::to_string(ctx->num_cu) +
+                             (phase_steps > 0 ? "/phase" + std::to_string(phase_steps) : std::string());
     if (cache_path && load_cache(ctx, cache_path, arch, max_G)) {
         if (n_buckets) *n_buckets = (int32_t)ctx->tiles.size();
         return SCRNA_OK;
@@ -190,7 +211,9 @@ int scrna_autotune(scrna_ctx* ctx, int32_t max_G, const char* cache_path, int32_
         for (int ch : {2, 4, 8, 16}) {
             if ((Gb + ch - 1) / ch > 16) continue;
             double ms = 0;
-            int rc = time_shape(ctx, s, ch, cells, 128, &ms);
+            // Whole 128-step launches, or eight launches of phase_steps steps.
+            const int steps = phase_steps > 0 ? 8 * phase_steps : 128;
+            int rc = time_shape(ctx, s, ch, cells, steps, phase_steps, &ms);
             if (rc) return rc;
             if (ms < best) { best = ms; best_ch = ch; }
         }

@@ -171,6 +171,7 @@ SIGNATURES = {
     "scrna_plan_use_graph": (C.c_int, [_VP]),
     "scrna_plan_destroy": (C.c_int, [_VP]),
     "scrna_autotune": (C.c_int, [_VP, C.c_int32, C.c_char_p, C.POINTER(C.c_int32)]),
+    "scrna_autotune_phases": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_char_p, C.POINTER(C.c_int32)]),
     "scrna_ctx_set_tile": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32]),
     "scrna_server_create": (C.c_int, [_VP, C.c_int32, C.POINTER(_VP)]),
     "scrna_submit": (C.c_int, [_VP, C.POINTER(ScrnaRequest), C.POINTER(C.c_int64)]),
@@ -247,11 +248,14 @@ class Context:
         check(self.lib.scrna_ctx_create(device, hbm_budget_bytes, C.byref(h)), "scrna_ctx_create")
         self.handle = h
 
-    def autotune(self, max_G: int = 256, cache_path: str | None = None) -> int:
-        """Pick genes-per-lane per G bucket by timing (scrna_autotune); returns buckets tuned."""
+    def autotune(self, max_G: int = 256, cache_path: str | None = None, phase_steps: int = 0) -> int:
+        """Pick genes-per-lane per G bucket by timing (scrna_autotune); returns buckets tuned.
+        phase_steps > 0 times shapes as population plans with that division interval run them
+        (scrna_autotune_phases)."""
         n = C.c_int32()
         path = cache_path.encode() if cache_path else None
-        check(self.lib.scrna_autotune(self.handle, int(max_G), path, C.byref(n)), "scrna_autotune")
+        check(self.lib.scrna_autotune_phases(self.handle, int(max_G), int(phase_steps), path, C.byref(n)),
+              "scrna_autotune_phases")
         return n.value
 
     def set_tile(self, G_lo: int, G_hi: int, genes_per_lane: int) -> None:

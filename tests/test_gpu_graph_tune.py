@@ -91,3 +91,20 @@ def test_autotune_writes_cache(tmp_path):
         assert ctx2.autotune(max_G=64, cache_path=path) == n      # cache hit
     finally:
         ctx2.close()
+
+
+def test_autotune_phases_keys_its_cache_by_launch_length(tmp_path):
+    """A table tuned for population phases is cached under its launch length and
+    is not taken for whole-horizon tuning (nor the other way round)."""
+    ctx2 = sc.Context(0)
+    try:
+        path = str(tmp_path / "tiles.json")
+        n = ctx2.autotune(max_G=16, cache_path=path, phase_steps=10)
+        saved = json.load(open(path))
+        assert saved["arch"].endswith("/phase10")
+        assert all(t[2] in (2, 4, 8, 16) for t in saved["table"]) and saved["table"][-1][1] >= 16
+        assert ctx2.autotune(max_G=16, cache_path=path, phase_steps=10) == n    # cache hit
+        ctx2.autotune(max_G=16, cache_path=path)                                 # different key: re-tuned
+        assert not json.load(open(path))["arch"].endswith("/phase10")
+    finally:
+        ctx2.close()
