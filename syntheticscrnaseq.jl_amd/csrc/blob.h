@@ -65,6 +65,7 @@ enum BlobHeader : int {
                       // gate weights by gate word [W], initial counts [G][2])
     H_TL_NEDGES,      // tau-leap: E
     H_TL_NGATEW,      // tau-leap: W (gate record words)
+    H_OFF_WREC,       // FP32 stepper with up to 4 genes per lane: [NW][rec_stride(CH)] wave records (RecWord)
     H_WORDS = 64
 };
 
@@ -97,6 +98,27 @@ enum BlobFlags : uint32_t {
     F_POP = 16u,      // population dynamics (volume-scaled transcription)
     F_NEG_INIT = 32u  // some initial count is negative (negative basal rates)
 };
+
+// Wave record of the FP32 stepper with up to 4 genes per lane: the values a
+// wave keeps in registers for the whole horizon, laid out by the host so that
+// a workgroup's start-up reads them with a few wide scalar loads.  Per gene
+// (kRecGeneWords words): the switch thresholds in units of 2^14 (0 when the
+// gene never switches), the byte offset of its Hill row pair in an LDS buffer
+// (sources only), its basal rate and kRecEdgeSlots in-edges as (byte offset of
+// the LDS row read, f32 weight), unused ones reading the buffer's zero row
+// with weight 0.  Then the wave's masks: switching genes, sources, and genes
+// with more in-edges (bits 0..15) or gates (bits 16..31).
+constexpr int kRecEdgeSlots = 2;
+enum RecWord : int {
+    R_TON = 0,
+    R_TOFF,
+    R_HROW,
+    R_BETA,
+    R_EDGE,           // kRecEdgeSlots (row offset, weight) pairs
+    R_GENE_WORDS = R_EDGE + 2 * kRecEdgeSlots
+};
+enum RecMask : int { RM_SWITCH = 0, RM_SOURCE, RM_EXTRA, RM_WORDS };
+constexpr int rec_stride(int CH) { return ((R_GENE_WORDS * CH + RM_WORDS + 15) / 16) * 16; }
 
 enum TeleWord : int {
     T_FLAGS = 0,      // bit0: switches (isfinite k_on), bit1: random init (both finite)

@@ -214,6 +214,7 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
     // then never flip; see cle_switch), and the LDS byte address of the
     // gene's Hill row pair in this lane (source genes only).
     uint32_t thr_on[CH], thr_off[CH], haddr[CH];
+    constexpr bool kVTab = CH <= 4;                        // set up from the wave record
     float beta_v[CH];                                      // basal rate of each gene
     // The first EK additive in-edges of each gene as (LDS byte address in this
     // lane's column, weight); unused slots read the buffer's zero row with
@@ -228,7 +229,30 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
     double V = 1.0;                                       // cell volume (population runs)
     const int S = (int)B[H_S];
     const uint32_t zaddr = (uint32_t)((2 * S * 64 + lane) * 4);   // zero row of a buffer, this lane
-    {
+    static_assert(kEdgeSlots4 == kRecEdgeSlots, "wave records hold the register edge slots");
+    if constexpr (kVTab) {
+        // The host laid these out per wave (RecWord): a few wide scalar loads.
+        const uint32_t* __restrict__ rec = B + B[H_OFF_WREC] + wave * rec_stride(CH);
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+            const uint32_t* __restrict__ rg = rec + k * R_GENE_WORDS;
+            thr_on[k] = rg[R_TON];
+            thr_off[k] = rg[R_TOFF];
+            haddr[k] = rg[R_HROW] + 4u * (uint32_t)lane;
+            beta_v[k] = as_f(rg[R_BETA]);
+#pragma unroll
+            for (int i = 0; i < EK; ++i) {
+                eaddr[k][i] = rg[R_EDGE + 2 * i] + 4u * (uint32_t)lane;
+                ew[k][i] = as_f(rg[R_EDGE + 2 * i + 1]);
+            }
+            asm volatile("" : "+v"(thr_on[k]), "+v"(thr_off[k]), "+v"(haddr[k]), "+v"(beta_v[k]));
+#pragma unroll
+            for (int i = 0; i < EK; ++i) asm volatile("" : "+v"(eaddr[k][i]), "+v"(ew[k][i]));
+        }
+        sw_mask = rec[R_GENE_WORDS * CH + RM_SWITCH];
+        src_mask = rec[R_GENE_WORDS * CH + RM_SOURCE];
+        extra_mask = rec[R_GENE_WORDS * CH + RM_EXTRA];
+    } else {
         const int32_t* __restrict__ slot_of = (const int32_t*)(B + B[H_OFF_SLOT]);
         const uint32_t* __restrict__ beta_w = B + B[H_OFF_BETA];
         const int32_t* __restrict__ in_off = (const int32_t*)(B + B[H_OFF_INOFF]);
@@ -285,7 +309,7 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
         on.set(k, true);
         if (g < G) {
             const uint32_t tf = telew[g * 8 + T_FLAGS];
-            sw_mask |= (tf & 1u) << k;
+            if (!kVTab) sw_mask |= (tf & 1u) << k;
             bool o = true;   // constitutive genes are always ON
             if (resume) {
                 m[k] = state_pool[off_state + (size_t)g * C + cell];

@@ -407,6 +407,7 @@ int compile_network(const scrna_network* net, const scrna_kinetics* kin, const s
         }
     align16();
     b[H_OFF_TELE] = (uint32_t)b.size();
+    std::vector<uint32_t> t18_on(G, 0u), t18_off(G, 0u), tflags(G, 0u);
     for (int g = 0; g < G; ++g) {
         uint32_t tf = 0;
         uint64_t thr_on = 0, thr_off = 0, thr_init = 0;
@@ -430,6 +431,44 @@ int compile_network(const scrna_network* net, const scrna_kinetics* kin, const s
         b.push_back((uint32_t)(thr_off >> 32));
         b.push_back((uint32_t)thr_init);
         b.push_back((uint32_t)(thr_init >> 32));
+        tflags[g] = tf;
+        if (tf & 1u) {
+            t18_on[g] = thr_on >> 32 ? 1u << 18 : (uint32_t)thr_on >> 14;
+            t18_off[g] = thr_off >> 32 ? 1u << 18 : (uint32_t)thr_off >> 14;
+        }
+    }
+    if (CH <= 4) {
+        // Wave records (RecWord), the values the stepper's waves hold in registers.
+        align16();
+        b[H_OFF_WREC] = (uint32_t)b.size();
+        const int stride = rec_stride(CH);
+        const uint32_t zero_row = (uint32_t)(2 * S * 64 * 4);
+        for (int w = 0; w < NW; ++w) {
+            std::vector<uint32_t> r((size_t)stride, 0u);
+            for (int k = 0; k < CH; ++k) {
+                const int g = w * CH + k;
+                uint32_t* rg = r.data() + (size_t)k * R_GENE_WORDS;
+                for (int i = 0; i < kRecEdgeSlots; ++i) rg[R_EDGE + 2 * i] = zero_row;
+                if (g >= G) continue;
+                rg[R_TON] = t18_on[g];
+                rg[R_TOFF] = t18_off[g];
+                rg[R_BETA] = f2w((float)net->basals[g]);
+                if (slot_of[g] >= 0) {
+                    rg[R_HROW] = (uint32_t)(2 * slot_of[g] * 64 * 4);
+                    r[R_GENE_WORDS * CH + RM_SOURCE] |= 1u << k;
+                }
+                for (int i = 0; i < kRecEdgeSlots && i < (int)in_edges[g].size(); ++i) {
+                    const auto& ed = in_edges[g][i];
+                    const int kind = ed.second > 0.0 ? 0 : 1;
+                    rg[R_EDGE + 2 * i] = (uint32_t)((2 * slot_of[ed.first] + kind) * 64 * 4);
+                    rg[R_EDGE + 2 * i + 1] = f2w((float)std::fabs(ed.second));
+                }
+                if ((int)in_edges[g].size() > kRecEdgeSlots) r[R_GENE_WORDS * CH + RM_EXTRA] |= 1u << k;
+                if (!gates_of[g].empty()) r[R_GENE_WORDS * CH + RM_EXTRA] |= 1u << (16 + k);
+                r[R_GENE_WORDS * CH + RM_SWITCH] |= (tflags[g] & 1u) << k;
+            }
+            b.insert(b.end(), r.begin(), r.end());
+        }
     }
     align16();
     b[H_OFF_INIT] = (uint32_t)b.size();
