@@ -503,16 +503,25 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
         }
     }
     if (live) {
+        // Output offsets re-read through an opaque blob offset rather than
+        // kept in scalar registers across the step loop.
+        uint32_t boff = blob_off;
+        asm("" : "+s"(boff) : "s"(s_stop));
+        const uint32_t* __restrict__ Be = blobs + boff;
+        const int Ge = (int)Be[H_G];
+        const size_t Ce = (size_t)Be[H_CELLS];
+        const size_t out_state = (size_t)u64_at(Be, H_OUT_FINAL_LO);
+        const size_t out_prom = (size_t)u64_at(Be, H_OUT_PROM_LO);
 #pragma unroll
         for (int k = 0; k < CH; ++k) {
             const int g = g0 + k;
-            if (g < G) {
-                state_pool[off_state + (size_t)g * C + cell] = m[k];
-                state_pool[off_state + (size_t)(G + g) * C + cell] = p[k];
-                prom_pool[off_prom + (size_t)g * C + cell] = (uint8_t)on.get(k);
+            if (g < Ge) {
+                state_pool[out_state + (size_t)g * Ce + cell] = m[k];
+                state_pool[out_state + (size_t)(Ge + g) * Ce + cell] = p[k];
+                prom_pool[out_prom + (size_t)g * Ce + cell] = (uint8_t)on.get(k);
             }
         }
-        if (pop && wave == 0) vol_pool[off_vol + cell] = V;
+        if (pop && wave == 0) vol_pool[(size_t)u64_at(Be, H_OUT_V_LO) + cell] = V;
     }
     segment_done(sched, sl);
 }
