@@ -208,6 +208,13 @@ int scrna_autotune_phases(scrna_ctx* ctx, int32_t max_G, int32_t phase_steps, co
         synth(Gb, s);
         int best_ch = 0;
         double best = 1e30;
+        // The default shape (4 genes per lane where 16 waves allow it) keeps
+        // the bucket unless another is faster by more than kTuneMargin: shapes
+        // within a few percent of each other otherwise trade places from run
+        // to run, and with them the tables.
+        constexpr double kTuneMargin = 0.03;
+        const int def_ch = (Gb + 3) / 4 <= 16 ? 4 : (Gb + 7) / 8 <= 16 ? 8 : 16;
+        double def_ms = 1e30;
         for (int ch : {2, 4, 8, 16}) {
             if ((Gb + ch - 1) / ch > 16) continue;
             double ms = 0;
@@ -215,8 +222,10 @@ int scrna_autotune_phases(scrna_ctx* ctx, int32_t max_G, int32_t phase_steps, co
             const int steps = phase_steps > 0 ? 8 * phase_steps : 128;
             int rc = time_shape(ctx, s, ch, cells, steps, phase_steps, &ms);
             if (rc) return rc;
+            if (ch == def_ch) def_ms = ms;
             if (ms < best) { best = ms; best_ch = ch; }
         }
+        if (def_ms < 1e30 && best > (1.0 - kTuneMargin) * def_ms) best_ch = def_ch;
         rules.push_back({prev + 1, Gb, best_ch});
         prev = Gb;
     }
