@@ -67,7 +67,14 @@ __device__ __forceinline__ void tl_gene_step(int& m, int& p, double lam_tx, doub
 // workgroup leave the FP64 samplers 256 VGPRs.
 constexpr int kTauLeapMaxWaves = 8;
 
-__global__ void __launch_bounds__(64 * kTauLeapMaxWaves)
+// Four waves per SIMD (128 VGPRs; the samplers' rarely used values spill to
+// scratch): a CU then holds four 4-wave workgroups instead of two, and the
+// extra ones fill the barrier and sampler-loop stalls of the others.  On the
+// G=20 benchmark network: 2 waves per SIMD (the compiler's own 199 VGPRs)
+// 205 ms, 3 waves 159.6 ms, 4 waves 153.6 ms.
+constexpr int kTauLeapWavesPerSimd = 4;
+
+__global__ void __launch_bounds__(64 * kTauLeapMaxWaves) __attribute__((amdgpu_waves_per_eu(kTauLeapWavesPerSimd)))
 tauleap_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ tiles,
                        float* __restrict__ state_pool, float* __restrict__ snap_pool,
                        uint8_t* __restrict__ prom_pool, double* __restrict__ vol_pool, int s_begin, int s_end,
