@@ -140,6 +140,10 @@ struct Promoters {
 // registers would spill.
 constexpr int kEdgeSlots4 = 2;
 constexpr int kWavesPerSimd4 = 7;
+// 8 genes per lane: 6 waves per SIMD (G=100, 50,000 cells, 2000 steps: 24.7
+// against 25.2 ms at the compiler's 5); 16 genes per lane keep the
+// compiler's choice (7 waves per SIMD spilled: G=200 telegraph 68 against 42 ms).
+constexpr int kWavesPerSimd8 = 6;
 //
 // INJ, POP and TELE say whether a launch may hold networks with injected
 // noise, a population or telegraph promoters; the flags in each blob still
@@ -163,7 +167,8 @@ constexpr int kWavesPerSimd4 = 7;
 // scalar cache and per-gene tests recomputed on the scalar unit instead of
 // being hoisted (booleans as lane masks) out of the loop.
 template <int CH, bool INJ, bool POP, bool TELE, bool CLAMP>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(CH <= 4 ? kWavesPerSimd4 : 1)))
+__global__ void __launch_bounds__(1024)
+__attribute__((amdgpu_waves_per_eu(CH <= 4 ? kWavesPerSimd4 : CH == 8 ? kWavesPerSimd8 : 1)))
 cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ tiles,
                    float* __restrict__ state_pool, float* __restrict__ snap_pool,
                    uint8_t* __restrict__ prom_pool, double* __restrict__ vol_pool,
