@@ -73,6 +73,13 @@ def load():
         lib.orc_cle_population.argtypes = [C.POINTER(OrcNetwork), C.POINTER(OrcKinetics), C.POINTER(OrcRun),
                                            C.POINTER(OrcPopulation), _dp, _dp, _dp, _up]
         lib.orc_cle_population.restype = C.c_int
+        lib.orc_cle_population_ex.argtypes = [C.POINTER(OrcNetwork), C.POINTER(OrcKinetics), C.POINTER(OrcRun),
+                                              C.POINTER(OrcPopulation), C.c_double, _dp, _dp, _dp, _up, _up]
+        lib.orc_cle_population_ex.restype = C.c_int
+        _lp = C.POINTER(C.c_int64)
+        lib.orc_divide.argtypes = [C.POINTER(OrcPopulation), C.c_int32, C.c_int64, C.c_uint64, C.c_uint32,
+                                   C.c_int64, _dp, _dp, _dp, _lp, _lp, _dp, _dp, _dp, _dp, _lp]
+        lib.orc_divide.restype = C.c_int
         lib.orc_tauleap_simulate.argtypes = lib.orc_cle_simulate.argtypes
         lib.orc_tauleap_simulate.restype = C.c_int
         lib.orc_tauleap_population.argtypes = lib.orc_cle_population.argtypes
@@ -209,8 +216,47 @@ def tauleap_population(net, kin, pop, *, dt, T, cell_num, seed=0, net_id=0, init
                           _fn="orc_tauleap_population")
 
 
+def _orc_pop(pop):
+    part = {"halve": 0, "binomial": 1}[pop.partition]
+    rep = {"uniform": 0, "exclude_mother": 1}[pop.replace]
+    return OrcPopulation(1, pop.growth_rate, pop.V_div, pop.V_init[0], pop.V_init[1], pop.div_check_interval,
+                         part, rep)
+
+
+def divide(pop, m_gc, p_gc, V, *, seed, step, net_id=0):
+    """One division check (population.jl:157-205) applied to a given state.
+
+    m_gc, p_gc: (G, N) arrays, V: (N,).  Returns a dict with the state after
+    the check (m, p, V) and the record of what it did: dividers (ascending),
+    mother_m/mother_p and daughter_m/daughter_p ((n_div, G), the mothers'
+    molecules after the partition and the daughters'), replaced (n_div,)."""
+    lib = load()
+    m = np.array(m_gc, dtype=np.float64, order="C", copy=True)
+    p = np.array(p_gc, dtype=np.float64, order="C", copy=True)
+    v = np.array(V, dtype=np.float64, copy=True)
+    G, N = m.shape
+    nd = C.c_int64()
+    div = np.empty(N, dtype=np.int64)
+    rep = np.empty(N, dtype=np.int64)
+    mm, mp, dm, dp = (np.empty((N, G)) for _ in range(4))
+    _lp = C.POINTER(C.c_int64)
+    rc = lib.orc_divide(C.byref(_orc_pop(pop)), G, N, seed & 0xFFFFFFFFFFFFFFFF, net_id, int(step),
+                        m.ctypes.data_as(_dp), p.ctypes.data_as(_dp), v.ctypes.data_as(_dp), C.byref(nd),
+                        div.ctypes.data_as(_lp), mm.ctypes.data_as(_dp), mp.ctypes.data_as(_dp),
+                        dm.ctypes.data_as(_dp), dp.ctypes.data_as(_dp), rep.ctypes.data_as(_lp))
+    if rc:
+        raise ValueError("oracle rejected the input")
+    k = nd.value
+    return dict(m=m, p=p, V=v, dividers=div[:k], mother_m=mm[:k], mother_p=mp[:k], daughter_m=dm[:k],
+                daughter_p=dp[:k], replaced=rep[:k])
+
+
 def cle_population(net, kin, pop, *, dt, T, cell_num, seed=0, net_id=0, init="steady", injected=None,
-                   _fn="orc_cle_population"):
+                   ambiguity=None, _fn="orc_cle_population"):
+    """FP64 population CLE (cle.jl:143-284 with population.jl:157-205).  With
+    ``ambiguity`` (a relative margin) the result also holds ``taint``: cells
+    whose binomial partition rounded a value within that margin of a
+    half-integer, or that hold a daughter of such a cell."""
     lib = load()
     n_steps = int(np.ceil(T / dt))
     s, keep = _net(net)
@@ -221,18 +267,25 @@ def cle_population(net, kin, pop, *, dt, T, cell_num, seed=0, net_id=0, init="st
         inj_p = inj.ctypes.data_as(_fp)
     run = OrcRun(dt, n_steps, cell_num, 0, cell_num, seed & 0xFFFFFFFFFFFFFFFF, net_id,
                  0 if init == "steady" else 1, 0, inj_p, 1)
-    part = {"halve": 0, "binomial": 1}[pop.partition]
-    rep = {"uniform": 0, "exclude_mother": 1}[pop.replace]
-    op = OrcPopulation(1, pop.growth_rate, pop.V_div, pop.V_init[0], pop.V_init[1], pop.div_check_interval,
-                       part, rep)
+    op = _orc_pop(pop)
     G = net.G
     m = np.empty((G, cell_num))
     p = np.empty((G, cell_num))
     V = np.empty(cell_num)
     pr = np.empty((G, cell_num), dtype=np.uint8)
+    if ambiguity is not None:
+        if _fn != "orc_cle_population":
+            raise ValueError("the ambiguity margin applies to the population CLE")
+        taint = np.zeros(cell_num, dtype=np.uint8)
+        rc = lib.orc_cle_population_ex(C.byref(s), C.byref(_kin(kin)), C.byref(run), C.byref(op),
+                                       float(ambiguity), m.ctypes.data_as(_dp), p.ctypes.data_as(_dp),
+                                       V.ctypes.data_as(_dp), pr.ctypes.data_as(_up), taint.ctypes.data_as(_up))
+        if rc:
+            raise ValueError("oracle rejected the input")
+        return dict(m=m, p=p, V=V, promoter=pr, taint=taint.astype(bool))
     rc = getattr(lib, _fn)(C.byref(s), C.byref(_kin(kin)), C.byref(run), C.byref(op),
-                                m.ctypes.data_as(_dp), p.ctypes.data_as(_dp), V.ctypes.data_as(_dp),
-                                pr.ctypes.data_as(_up))
+                           m.ctypes.data_as(_dp), p.ctypes.data_as(_dp), V.ctypes.data_as(_dp),
+                           pr.ctypes.data_as(_up))
     if rc:
         raise ValueError("oracle rejected the input")
     return dict(m=m, p=p, V=V, promoter=pr)

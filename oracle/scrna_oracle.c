@@ -575,14 +575,41 @@ int orc_capture(const double* X, int64_t cells, int32_t G, int64_t ld, double ef
     return 0;
 }
 
+/* What one division check did, for the tests (any pointer may be NULL):
+ * n_div dividers in ascending cell order, the mothers' molecules after
+ * partitioning and the daughters' (cell-major [k*G + g]), the slot each
+ * daughter replaced, and the ambiguity taint (below) per cell. */
+typedef struct orc_div_record {
+    int64_t n_div;
+    int64_t* dividers;            /* [N] */
+    double *mother_m, *mother_p;  /* [N*G] */
+    double *daughter_m, *daughter_p;
+    int64_t* replaced;            /* [N] */
+} orc_div_record;
+
+/* Julia's round half to even, and whether x lies within amb * max(1, |x|) of
+ * a rounding boundary (a half-integer): a restatement in other arithmetic
+ * (the GPU's FP32 state) may round such a value the other way. */
+static inline int orc_round_ambiguous(double x, double amb) {
+    if (!(amb > 0.0)) return 0;
+    const double f = x - floor(x);
+    const double scale = fabs(x) > 1.0 ? fabs(x) : 1.0;
+    return fabs(f - 0.5) < amb * scale;
+}
+
 /* One division check (population.jl:157-205; cle.jl:240-272): every cell
  * with V > V_div divides, then each daughter replaces a cell (Moran).
- * Daughter buffers are caller-provided ([N] and [N*G]). */
+ * Daughter buffers are caller-provided ([N] and [N*G]).  With amb > 0 and
+ * taint != NULL, a divider whose binomial partition rounds a value within
+ * amb of a half-integer is marked, and a replaced slot takes its daughter's
+ * mark (the mother's, so marks follow the lineage). */
 static void orc_division_check(const orc_population* pop, int G, int64_t N, uint64_t seed, uint32_t c3base,
                                int64_t s, double* M, double* P, double* V, uint8_t* on, int64_t* div_idx,
-                               double* dM, double* dP, double* dV, uint8_t* dOn) {
+                               double* dM, double* dP, double* dV, uint8_t* dOn, double amb, uint8_t* taint,
+                               orc_div_record* rec) {
     int64_t nd = 0;
     for (int64_t c = 0; c < N; ++c) if (V[c] > pop->V_div) div_idx[nd++] = c;   /* cle.jl:241 */
+    uint8_t* dtaint = taint ? (uint8_t*)calloc((size_t)(nd > 0 ? nd : 1), 1) : NULL;
     /* Phase 1: partition mothers, buffer daughters (population.jl:166-192) */
     for (int64_t k = 0; k < nd; ++k) {
         int64_t c = div_idx[k];
@@ -593,6 +620,7 @@ static void orc_division_check(const orc_population* pop, int G, int64_t N, uint
                 if (pop->partition == 0) {                      /* cle.jl:251-257 */
                     double h = *X / 2.0; *X = h; *D = h;
                 } else {
+                    if (taint && orc_round_ambiguous(*X, amb)) taint[c] = 1;
                     int64_t n = orc_round_even(*X);
                     if (n < 0) n = 0;
                     int64_t d = 0;
@@ -610,6 +638,16 @@ static void orc_division_check(const orc_population* pop, int G, int64_t N, uint
         }
         V[c] /= 2.0;                                            /* population.jl:190 */
         dV[k] = V[c];
+        if (dtaint) dtaint[k] = taint[c];
+        if (rec) {
+            rec->dividers[k] = c;
+            for (int g = 0; g < G; ++g) {
+                rec->mother_m[k * G + g] = M[c * G + g];
+                rec->mother_p[k * G + g] = P[c * G + g];
+                rec->daughter_m[k * G + g] = dM[k * G + g];
+                rec->daughter_p[k * G + g] = dP[k * G + g];
+            }
+        }
     }
     /* Phase 2: Moran replacement in divider order (population.jl:194-202) */
     for (int64_t k = 0; k < nd; ++k) {
@@ -630,7 +668,47 @@ static void orc_division_check(const orc_population* pop, int G, int64_t N, uint
             on[replaced * G + g] = dOn[k * G + g];
         }
         V[replaced] = dV[k];
+        if (taint) taint[replaced] = dtaint[k];
+        if (rec) rec->replaced[k] = replaced;
     }
+    if (rec) rec->n_div = nd;
+    free(dtaint);
+}
+
+int orc_divide(const orc_population* pop, int32_t G, int64_t N, uint64_t seed, uint32_t net_id, int64_t step,
+               double* m, double* p, double* V, int64_t* n_div, int64_t* dividers, double* mother_m,
+               double* mother_p, double* daughter_m, double* daughter_p, int64_t* replaced) {
+    if (!pop || G <= 0 || N <= 0 || !m || !p || !V) return 1;
+    double* M = (double*)malloc(sizeof(double) * (size_t)G * (size_t)N);
+    double* P = (double*)malloc(sizeof(double) * (size_t)G * (size_t)N);
+    uint8_t* on = (uint8_t*)calloc((size_t)G * (size_t)N, 1);
+    uint8_t* dOn = (uint8_t*)calloc((size_t)G * (size_t)N, 1);
+    double* dM = (double*)malloc(sizeof(double) * (size_t)G * (size_t)N);
+    double* dP = (double*)malloc(sizeof(double) * (size_t)G * (size_t)N);
+    double* dV = (double*)malloc(sizeof(double) * (size_t)N);
+    int64_t* div_idx = (int64_t*)malloc(sizeof(int64_t) * (size_t)N);
+    int64_t* rdiv = dividers ? dividers : (int64_t*)malloc(sizeof(int64_t) * (size_t)N);
+    int64_t* rrep = replaced ? replaced : (int64_t*)malloc(sizeof(int64_t) * (size_t)N);
+    double* scratch = (double*)malloc(sizeof(double) * (size_t)G * (size_t)N * 4);
+    orc_div_record rec = {0, rdiv, mother_m ? mother_m : scratch, mother_p ? mother_p : scratch + (size_t)G * N,
+                          daughter_m ? daughter_m : scratch + 2 * (size_t)G * N,
+                          daughter_p ? daughter_p : scratch + 3 * (size_t)G * N, rrep};
+    for (int64_t c = 0; c < N; ++c)
+        for (int g = 0; g < G; ++g) {
+            M[c * G + g] = m[(size_t)g * N + c];
+            P[c * G + g] = p[(size_t)g * N + c];
+        }
+    orc_division_check(pop, G, N, seed, net_id << 4, step, M, P, V, on, div_idx, dM, dP, dV, dOn, 0.0, NULL, &rec);
+    for (int64_t c = 0; c < N; ++c)
+        for (int g = 0; g < G; ++g) {
+            m[(size_t)g * N + c] = M[c * G + g];
+            p[(size_t)g * N + c] = P[c * G + g];
+        }
+    if (n_div) *n_div = rec.n_div;
+    if (!dividers) free(rdiv);
+    if (!replaced) free(rrep);
+    free(M); free(P); free(on); free(dOn); free(dM); free(dP); free(dV); free(div_idx); free(scratch);
+    return 0;
 }
 
 /* ---------------------------------------------------------------------- */
@@ -639,6 +717,12 @@ static void orc_division_check(const orc_population* pop, int G, int64_t N, uint
 int orc_cle_population(const orc_network* net, const orc_kinetics* kin, const orc_run* run,
                        const orc_population* pop, double* m_out, double* p_out, double* V_out,
                        uint8_t* promoter_out) {
+    return orc_cle_population_ex(net, kin, run, pop, 0.0, m_out, p_out, V_out, promoter_out, NULL);
+}
+
+int orc_cle_population_ex(const orc_network* net, const orc_kinetics* kin, const orc_run* run,
+                          const orc_population* pop, double amb, double* m_out, double* p_out, double* V_out,
+                          uint8_t* promoter_out, uint8_t* taint_out) {
     if (orc_check(net, kin, run) || !pop || pop->div_check_interval <= 0) return 1;
     const int G = net->G;
     const int64_t N = run->cell_num;
@@ -661,6 +745,7 @@ int orc_cle_population(const orc_network* net, const orc_kinetics* kin, const or
     double* dP = (double*)malloc(sizeof(double) * (size_t)G * (size_t)N);
     double* dV = (double*)malloc(sizeof(double) * (size_t)N);
     uint8_t* dOn = (uint8_t*)malloc((size_t)G * (size_t)N);
+    uint8_t* taint = taint_out ? (uint8_t*)calloc((size_t)N, 1) : NULL;
 
     for (int64_t c = 0; c < N; ++c) {
         for (int g = 0; g < G; ++g) {
@@ -713,7 +798,8 @@ int orc_cle_population(const orc_network* net, const orc_kinetics* kin, const or
             V[c] *= growth;                                             /* cle.jl:237 */
         }
         if (((s + 1) % pop->div_check_interval) == 0)                   /* cle.jl:240 */
-            orc_division_check(pop, G, N, run->seed, c3base, s, M, P, V, on, div_idx, dM, dP, dV, dOn);
+            orc_division_check(pop, G, N, run->seed, c3base, s, M, P, V, on, div_idx, dM, dP, dV, dOn, amb, taint,
+                               NULL);
     }
     for (int64_t c = 0; c < N; ++c) {
         for (int g = 0; g < G; ++g) {
@@ -722,9 +808,10 @@ int orc_cle_population(const orc_network* net, const orc_kinetics* kin, const or
             if (promoter_out) promoter_out[(size_t)g * N + c] = on[c * G + g];
         }
         if (V_out) V_out[c] = V[c];
+        if (taint_out) taint_out[c] = taint[c];
     }
     free(M); free(P); free(V); free(on); free(reg); free(pcol); free(div_idx);
-    free(dM); free(dP); free(dV); free(dOn);
+    free(dM); free(dP); free(dV); free(dOn); free(taint);
     orc_tele_free(&tele);
     return 0;
 }
@@ -903,7 +990,8 @@ int orc_tauleap_population(const orc_network* net, const orc_kinetics* kin, cons
         }
         for (int64_t c = 0; c < N; ++c) V[c] *= growth;                         /* :253; population.jl:136-137 */
         if (((s + 1) % pop->div_check_interval) == 0)                            /* :256-258 */
-            orc_division_check(pop, G, N, run->seed, c3base, s, M, P, V, on, div_idx, dM, dP, dV, dOn);
+            orc_division_check(pop, G, N, run->seed, c3base, s, M, P, V, on, div_idx, dM, dP, dV, dOn, 0.0, NULL,
+                               NULL);
     }
     for (int64_t c = 0; c < N; ++c) {
         for (int g = 0; g < G; ++g) {

@@ -117,6 +117,27 @@ int orc_cle_population(const orc_network* net, const orc_kinetics* kin, const or
                        const orc_population* pop, double* m_out, double* p_out,
                        double* V_out, uint8_t* promoter_out);
 
+/* The same with an ambiguity margin: taint_out[c] (optional, [cell_num])
+ * is 1 for a cell one of whose values rounded at a binomial partition lay
+ * within amb * max(1, |x|) of a half-integer, or that holds (by Moran
+ * replacement) a daughter of such a cell.  An FP32 restatement whose state
+ * stays within that margin of this oracle's partitions every other cell
+ * identically. */
+int orc_cle_population_ex(const orc_network* net, const orc_kinetics* kin, const orc_run* run,
+                          const orc_population* pop, double amb, double* m_out, double* p_out,
+                          double* V_out, uint8_t* promoter_out, uint8_t* taint_out);
+
+/* One division check (population.jl:157-205) on a given state, for tests.
+ * m, p: [G*N] gene-major (element (g, c) at g*N + c), updated in place; V:
+ * [N] updated in place; promoter states are not modelled.  Outputs (each
+ * optional): *n_div, dividers [N] (ascending), mothers' molecules after the
+ * partition and daughters' ([n_div*G], divider-major), and the slot each
+ * daughter replaced [N].  `step` is the 0-based step whose end the check
+ * follows (it keys the partition and replacement draws). */
+int orc_divide(const orc_population* pop, int32_t G, int64_t N, uint64_t seed, uint32_t net_id, int64_t step,
+               double* m, double* p, double* V, int64_t* n_div, int64_t* dividers, double* mother_m,
+               double* mother_p, double* daughter_m, double* daughter_p, int64_t* replaced);
+
 /* Binomial tau-leap of cells [cell_lo, cell_hi) (tauleap_binomial.jl:31-180):
  * same layouts as orc_cle_simulate; counts are returned as doubles.  init_mode
  * 0 rounds the steady state half to even (tauleap_binomial.jl:61-62).

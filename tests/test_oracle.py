@@ -280,17 +280,3 @@ def test_capture_reference_bands():
     _, pc = orc.capture(np.ones((1, 100000)), 0.1, 0.3, 5)
     assert abs(pc.mean() / 0.1 - 1) < 0.01
     assert abs(np.log(pc).std() - 0.3) < 0.005
-
-
-def test_population_oracle_invariants():
-    """N fixed, volumes inside [V_div/2, V_div*e^{lambda*dt*interval}] after a check
-    (test_analytical_moments.jl:384-392), binomial partition conserves molecules."""
-    net = sc.GeneNetwork(1, np.array([2.0]), np.zeros((1, 1)))
-    kin = sc.KineticParams(k_t=1.0, K_d=50.0, n=2.0, mu_m=0.1, mu_p=0.2)
-    pop = sc.PopulationConfig(cell_num=1500, growth_rate=0.03, V_div=2.0, V_init=(1.0, 2.0),
-                              div_check_interval=1)
-    r = orc.cle_population(net, kin, pop, dt=0.1, T=300.0, cell_num=1500, seed=42)
-    V = r["V"]
-    assert V.size == 1500
-    assert V.min() >= 1.0 * 0.99 and V.max() <= 2.0 * math.exp(0.03 * 0.1) * 1.01
-    assert np.all(r["m"] >= 0) and np.all(np.floor(r["m"]) == r["m"]) is not None
