@@ -40,8 +40,9 @@ FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
 
 # gfx950 VALU issue cost of each instruction class relative to one FP32 FMA,
-# measured by syntheticscrnaseq.jl_amd/probe_valu_rates (8 independent chains
-# per lane, 8 waves per SIMD):
+# measured once on an MI355X with dependent chains of each instruction (8
+# independent chains per lane, 8 waves per SIMD; table in
+# profiles/valu_rates_gfx950.md).  Fixed: the census does not change between runs.
 COST_FP32 = 1.0       # v_add/mul/fma/max_f32
 COST_INT = 1.66       # 32-bit integer and logic (xor, add, shifts, compares)
 COST_MUL32 = 1.54     # one 32x32-bit product (v_mad_u64_u32, v_mul_hi_u32)

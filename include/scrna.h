@@ -100,8 +100,8 @@ typedef struct scrna_population {
     double V_div;
     double V_init_lo, V_init_hi;
     int32_t div_check_interval;
-    int32_t partition;            /* 0: halve (cle.jl:249-261, ext:494-505); 1: Binomial(round(x), 1/2) (population.jl:423-440) */
-    int32_t replace_mode;         /* 0: uniform over 1:N (population.jl:448, ext:509); 1: exclude mother (cle.jl:265-268) */
+    int32_t partition;            /* 0: halve (cle.jl:249-261, ext:494-505); 1: Binomial(round(x), 1/2) (population.jl:166-192) */
+    int32_t replace_mode;         /* 0: uniform over 1:N (population.jl:196, ext:509); 1: exclude mother (cle.jl:265-268) */
 } scrna_population;
 
 /* CaptureModel (population.jl:57-74). */

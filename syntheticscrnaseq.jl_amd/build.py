@@ -23,8 +23,6 @@ OBJDIR = os.path.join(HERE, "_build")
 SOURCES = ["cle_kernels.hip", "runtime.cpp", "population.cpp", "server.cpp", "dense_kernels.hip",
            "autotune.cpp", "unfused.hip", "tauleap_kernels.hip", "prior.cpp",
            "mtx_writer.cpp"]
-# Stand-alone measurement programs (not linked into the library).
-PROBES = ["probe_valu_rates.hip"]
 
 COMMON = [
     "--offload-arch=gfx950",
@@ -71,12 +69,6 @@ def build(jobs: int = 4) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}{r.stderr}")
     os.replace(tmp, OUT)
-    for src in PROBES:
-        exe = os.path.join(HERE, os.path.splitext(src)[0])
-        cmd = [hipcc(), *COMMON, os.path.join(CSRC, src), "-o", exe]
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        if r.returncode != 0:
-            raise RuntimeError(f"hipcc failed on {src}:\n{r.stdout}{r.stderr}")
     return OUT
 
 

@@ -609,7 +609,7 @@ moments_kernel(const MomentJob* __restrict__ jobs, const float* __restrict__ poo
 // ---------------------------------------------------------------------------
 // Population dynamics (Moran process), one thread per cell of a tile.
 //
-// pop_init_kernel: V ~ U(V_init_lo, V_init_hi) (cle.jl:158-159, population.jl:364-365).
+// pop_init_kernel: V ~ U(V_init_lo, V_init_hi) (cle.jl:158-159, population.jl:112-113).
 __global__ void __launch_bounds__(64)
 pop_init_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ tiles, double* __restrict__ vol_pool,
                 const double* __restrict__ vinit /* [2*n_nets] lo, hi */) {

@@ -90,14 +90,21 @@ int ctx_alloc(scrna_ctx* ctx, void** p, size_t bytes) {
         *p = nullptr;
         return SCRNA_OK;
     }
-    size_t used = ctx->used.load();
-    if (used + bytes > ctx->budget)
-        return set_error(SCRNA_ERR_BUDGET, "HBM budget exceeded: need %zu more bytes, %zu of %zu in use",
-                         bytes, used, (size_t)ctx->budget);
-    HIP_TRY(hipSetDevice(ctx->device));
-    hipError_t e = hipMalloc(p, bytes);
-    if (e != hipSuccess) return set_error(SCRNA_ERR_DEVICE, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
-    ctx->used += bytes;
+    // Reserve the bytes against the budget first (compare-exchange, so that
+    // concurrent callers cannot both pass the check), then allocate.
+    uint64_t used = ctx->used.load();
+    do {
+        if (used + bytes > ctx->budget)
+            return set_error(SCRNA_ERR_BUDGET, "HBM budget exceeded: need %zu more bytes, %zu of %zu in use",
+                             bytes, (size_t)used, (size_t)ctx->budget);
+    } while (!ctx->used.compare_exchange_weak(used, used + bytes));
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        ctx->used -= bytes;
+        *p = nullptr;
+        return set_error(SCRNA_ERR_DEVICE, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
     return SCRNA_OK;
 }
 
@@ -576,7 +583,8 @@ void plan_release(scrna_plan* p) {
     p->allocs.clear();
     if (p->ev_start) (void)hipEventDestroy(p->ev_start);
     if (p->ev_stop) (void)hipEventDestroy(p->ev_stop);
-    p->ev_start = p->ev_stop = nullptr;
+    if (p->ev_done) (void)hipEventDestroy(p->ev_done);
+    p->ev_start = p->ev_stop = p->ev_done = nullptr;
     for (int i = 0; i < scrna_plan::kSideStreams; ++i) {
         if (p->side[i]) {
             (void)hipStreamSynchronize(p->side[i]);
@@ -902,7 +910,8 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
         plan_release(P.get());
         return set_error(SCRNA_ERR_DEVICE, "upload sync failed: %s", hipGetErrorString(e));
     }
-    if (hipEventCreate(&P->ev_start) != hipSuccess || hipEventCreate(&P->ev_stop) != hipSuccess) {
+    if (hipEventCreate(&P->ev_start) != hipSuccess || hipEventCreate(&P->ev_stop) != hipSuccess ||
+        hipEventCreateWithFlags(&P->ev_done, hipEventDisableTiming) != hipSuccess) {
         plan_release(P.get());
         return set_error(SCRNA_ERR_DEVICE, "hipEventCreate failed");
     }
@@ -1034,6 +1043,7 @@ int plan_launch(scrna_plan* P, hipStream_t st) {
         int rc = plan_enqueue(P, st, true);
         if (rc) return rc;
     }
+    HIP_TRY(hipEventRecord(P->ev_done, st));
     P->launched = true;
     return SCRNA_OK;
 }
@@ -1075,6 +1085,7 @@ int plan_download(scrna_plan* P, int32_t i, scrna_output* out) {
     const CompiledNet& cn = P->nets[i];
     const size_t C = (size_t)cn.cells, G = (size_t)cn.G, GC = G * C;
     HIP_TRY(hipSetDevice(P->ctx->device));
+    if (P->launched) HIP_TRY(hipEventSynchronize(P->ev_done));   // the copies below use the null stream
     if (out->y) {
         const int64_t ld = out->ld > 0 ? out->ld : (int64_t)C;
         if (ld < (int64_t)C) return set_error(SCRNA_ERR_INVALID, "output leading dimension %lld < cells", (long long)ld);
@@ -1197,8 +1208,10 @@ int scrna_plan_launch(scrna_plan* plan, void* stream) {
 int scrna_plan_sync(scrna_plan* plan) {
     if (!plan) return set_error(SCRNA_ERR_INVALID, "null plan");
     HIP_TRY(hipSetDevice(plan->ctx->device));
-    HIP_TRY(hipEventSynchronize(plan->ev_stop));
-    HIP_TRY(hipDeviceSynchronize());
+    // The launch stream's last event covers the side lanes too (they join it
+    // before the moments and capture kernels): other work on the device is
+    // not waited for.
+    HIP_TRY(hipEventSynchronize(plan->ev_done));
     for (const auto& lg : plan->groups) {
         if (lg.kind == 1 || lg.nseg <= 1) continue;
         int err = 0;
@@ -1239,7 +1252,7 @@ int scrna_plan_use_graph(scrna_plan* plan) {
 int scrna_plan_destroy(scrna_plan* plan) {
     if (!plan) return SCRNA_OK;
     (void)hipSetDevice(plan->ctx->device);
-    (void)hipDeviceSynchronize();
+    if (plan->launched && plan->ev_done) (void)hipEventSynchronize(plan->ev_done);
     plan_release(plan);
     delete plan;
     return SCRNA_OK;

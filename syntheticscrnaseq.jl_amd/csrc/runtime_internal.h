@@ -112,6 +112,7 @@ struct scrna_plan {
     int64_t max_cap_elems = 0;
     int max_mom_G = 0;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    hipEvent_t ev_done = nullptr;         // after the last kernel of a launch (sync and destroy wait on it)
     // Launch groups run side by side: lanes > 0 fork from the launch stream
     // at ev_fork and join back through ev_join (also inside graph capture).
     static constexpr int kSideStreams = 3;

@@ -36,6 +36,10 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
+// Budget left beside the arenas for the server's scratch: tile tables,
+// schedule blocks and job tables (a few KB per admitted request).
+constexpr uint64_t kScratchReserve = 32ull << 20;
+
 // First-fit range allocator over [0, size) in 256-byte granules.
 class RangeAlloc {
    public:
@@ -203,13 +207,16 @@ bool admit(scrna_server* S, Req& r) {
     return true;
 }
 
+// Grow a scratch buffer (tiles, schedule blocks, job tables) to hold `need`
+// elements.  Scratch is charged to the context's HBM budget like the arenas;
+// scrna_server_create leaves kScratchReserve of the budget for it.
 bool ensure_dev(scrna_server* S, void** p, size_t* cap, size_t need, size_t elem) {
     if (need <= *cap) return true;
     size_t n = std::max(need, *cap * 2);
-    if (*p) (void)hipFree(*p);
+    ctx_free(S->ctx, *p, *cap * elem);
     *p = nullptr;
     *cap = 0;
-    if (hipMalloc(p, n * elem) != hipSuccess) return false;
+    if (ctx_alloc(S->ctx, p, n * elem) != SCRNA_OK) return false;
     *cap = n;
     return true;
 }
@@ -249,11 +256,20 @@ void finish(scrna_server* S, std::vector<Req*>& done) {
     hipError_t e = hipSuccess;
     // job buffers are sized for jobs_cap entries of the larger struct
     if (mj.size() > S->jobs_cap || cj.size() > S->jobs_cap) {
-        if (S->d_mjobs) (void)hipFree(S->d_mjobs);
-        if (S->d_cjobs) (void)hipFree(S->d_cjobs);
-        S->jobs_cap = std::max(mj.size(), cj.size()) * 2;
-        e = hipMalloc((void**)&S->d_mjobs, S->jobs_cap * sizeof(MomentJob));
-        if (e == hipSuccess) e = hipMalloc((void**)&S->d_cjobs, S->jobs_cap * sizeof(CaptureJob));
+        ctx_free(S->ctx, S->d_mjobs, S->jobs_cap * sizeof(MomentJob));
+        ctx_free(S->ctx, S->d_cjobs, S->jobs_cap * sizeof(CaptureJob));
+        S->d_mjobs = nullptr;
+        S->d_cjobs = nullptr;
+        const size_t cap = std::max(mj.size(), cj.size()) * 2;
+        S->jobs_cap = 0;
+        if (ctx_alloc(S->ctx, (void**)&S->d_mjobs, cap * sizeof(MomentJob)) != SCRNA_OK ||
+            ctx_alloc(S->ctx, (void**)&S->d_cjobs, cap * sizeof(CaptureJob)) != SCRNA_OK) {
+            ctx_free(S->ctx, S->d_mjobs, cap * sizeof(MomentJob));
+            S->d_mjobs = nullptr;
+            e = hipErrorOutOfMemory;
+        } else {
+            S->jobs_cap = cap;
+        }
     }
     float* fpool = (float*)S->arena;
     if (e == hipSuccess)
@@ -518,10 +534,13 @@ int scrna_server_create(scrna_ctx* ctx, int32_t max_slots, scrna_server** out) {
     std::unique_ptr<scrna_server> S(new scrna_server());
     S->ctx = ctx;
     S->max_slots = max_slots;
-    const uint64_t room = ctx->budget > ctx->used ? ctx->budget - ctx->used : 0;
+    // Plans and servers of this context size themselves from the budget under ctx->mu.
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const uint64_t used = ctx->used.load();
+    const uint64_t room = ctx->budget > used ? ctx->budget - used : 0;
     S->blob_words = 16u << 20;   // 64 MiB of network descriptors
-    S->arena_bytes = (size_t)std::min<uint64_t>(room > (uint64_t)S->blob_words * 4 ? room - S->blob_words * 4 : 0,
-                                                (uint64_t)64 << 30);
+    const uint64_t fixed = (uint64_t)S->blob_words * 4 + kScratchReserve;
+    S->arena_bytes = (size_t)std::min<uint64_t>(room > fixed ? room - fixed : 0, (uint64_t)64 << 30);
     S->arena_bytes &= ~(size_t)255;
     if (S->arena_bytes < (1u << 20)) return set_error(SCRNA_ERR_BUDGET, "HBM budget leaves no room for a serving arena");
     int rc = ctx_alloc(ctx, &S->arena, S->arena_bytes);
@@ -679,10 +698,10 @@ int scrna_server_destroy(scrna_server* S) {
     if (S->worker.joinable()) S->worker.join();
     (void)hipSetDevice(S->ctx->device);
     destroy_streams(S);
-    if (S->d_tiles) (void)hipFree(S->d_tiles);
-    if (S->d_sched) (void)hipFree(S->d_sched);
-    if (S->d_mjobs) (void)hipFree(S->d_mjobs);
-    if (S->d_cjobs) (void)hipFree(S->d_cjobs);
+    ctx_free(S->ctx, S->d_tiles, S->tiles_cap * sizeof(Tile));
+    ctx_free(S->ctx, S->d_sched, S->sched_cap * sizeof(int));
+    ctx_free(S->ctx, S->d_mjobs, S->jobs_cap * sizeof(MomentJob));
+    ctx_free(S->ctx, S->d_cjobs, S->jobs_cap * sizeof(CaptureJob));
     ctx_free(S->ctx, S->arena, S->arena_bytes);
     ctx_free(S->ctx, S->blob_arena, S->blob_words * 4);
     delete S;
