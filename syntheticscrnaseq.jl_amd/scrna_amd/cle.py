@@ -226,11 +226,14 @@ def apply_capture(X, capture: CaptureModel, *, seed=None, net_id: int = 0,
 
 
 class Plan:
-    """Device-resident batch: upload once, launch many times (scrna_plan_*)."""
+    """Device-resident batch: upload once, launch many times (scrna_plan_*).
+
+    Arguments as simulate_gpu_batch; network i draws the stream (seed, net_id=i).
+    The plan copies ``injected_noise`` at creation."""
 
     def __init__(self, networks, alg, kin, *, cell_num=1000, T=50.0, readout="protein",
                  population=None, capture=None, seed=None, init="steady", snapshot_every=0,
-                 precision="fp32", ctx: _lib.Context | None = None):
+                 precision="fp32", injected_noise=None, ctx: _lib.Context | None = None):
         nets = list(networks)
         n = len(nets)
         self.ctx = ctx or _lib.default_context()
@@ -240,10 +243,11 @@ class Plan:
         Ts = _listify(T, n)
         pops = _listify(population, n)
         caps = _listify(capture, n)
+        injs = _listify(injected_noise, n) if isinstance(injected_noise, (list, tuple)) else [injected_noise] * n
         s = _seed(seed)
         runs_kw = [dict(alg=alg, cell_num=(pops[i].cell_num if pops[i] is not None else cells[i]), T=Ts[i],
                         readout=readout, init=init, seed=s, net_id=i, precision=precision,
-                        snapshot_every=snapshot_every, injected=None) for i in range(n)]
+                        snapshot_every=snapshot_every, injected=injs[i]) for i in range(n)]
         cn, ck, cr, cp, cc, keep = _prepare(nets, kins, runs_kw, pops, caps)
         h = C.c_void_p()
         _lib.check(self.lib.scrna_plan_create(self.ctx.handle, n, cn, ck, cr, cp, cc, C.byref(h)),

@@ -4,8 +4,9 @@ division rules of population.jl:157-205).
 Volumes are double precision on both sides and grow by the same host-computed
 factor, so division decisions and replacement slots agree exactly; the
 molecule state then agrees to the FP32 tolerance (halving) or, for binomial
-partitioning, for all cells whose rounding at division did not straddle a
-half-integer.
+partitioning, on every cell the oracle does not mark as ambiguous (a value
+rounded at a partition within 1e-5 of a half-integer, or a daughter of such
+a cell).
 """
 import numpy as np
 import pytest
@@ -36,15 +37,20 @@ def test_population_matches_oracle(ctx, partition, replace):
     print(f"population {partition}/{replace} cells={cells} steps={steps}", flush=True)
     res = sc.simulate(net, sc.CLE(dt), KIN, population=pop, T=T, readout="both", seed=9, injected_noise=xi,
                       full=True)
-    ref = orc.cle_population(net, KIN, pop, dt=dt, T=T, cell_num=cells, seed=9, injected=xi)
+    ref = orc.cle_population(net, KIN, pop, dt=dt, T=T, cell_num=cells, seed=9, injected=xi, ambiguity=1e-5)
     np.testing.assert_array_equal(res.volumes, ref["V"])
     want = np.concatenate([ref["m"].T, ref["p"].T], axis=1)
     err = np.abs(res.y - want) / np.maximum(np.abs(want), 1.0)
     per_cell = err.max(axis=1)
+    # Halving has no rounding: every cell within 1e-4.  Binomial partitioning
+    # rounds the FP32 state; only cells whose rounded values lay within 1e-5 of
+    # a half-integer (and the slots holding their daughters) may differ.
+    clean = ~ref["taint"]
     if partition == "halve":
-        assert per_cell.max() <= 1e-4
-    else:
-        assert np.mean(per_cell <= 1e-4) >= 0.95
+        assert clean.all()
+    print(f"{int(ref['taint'].sum())} of {cells} cells marked ambiguous", flush=True)
+    assert per_cell[clean].max() <= 1e-4
+    assert ref["taint"].mean() <= 0.10
     assert res.y.shape == (cells, 2 * net.G)
 
 
