@@ -4,8 +4,9 @@ entry point the Julia shim binds, called directly through ctypes.
 * The CPU reference workload (G=5 mixed network, constitutive, 5000 cells
   from zero counts, dt=0.01, T=50, LogNormal(0.1, 0.3)-Binomial capture) at
   full size: every cell against the FP64 oracle replaying the same Philox
-  stream (5e-5: GPU FP32 normals against the FP64 transform of the same u32), the fused capture bit-exact against the oracle's capture of
-  the GPU state, and the per-gene means against the oracle's within 0.5%.
+  stream (5e-5: GPU FP32 normals against the FP64 transform of the same
+  u32), the fused capture bit-exact against the oracle's capture of the GPU
+  state, and the per-gene means against the oracle's within 0.5%.
 * The serving workload's request mix (G in {10, 50, 200}, half telegraph,
   capture, horizons up to T=200 at dt=0.01) at reduced cell counts with
   Poisson arrivals: each served result equals the same request alone, and
