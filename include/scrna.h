@@ -168,6 +168,26 @@ int scrna_simulate_cle(scrna_ctx* ctx, const scrna_network* net, const scrna_kin
                        const scrna_run* run, const scrna_population* pop,
                        const scrna_capture* capture, scrna_output* out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation): scrna_simulate_cle for one network without population
+ * dynamics (there is no population argument: population runs need daughter
+ * and claim buffers), allocating no device memory.  Every device buffer of the run --
+ * network descriptor, tiles, schedule block, job tables, state, promoter and
+ * snapshot pools, counts, moments, injected noise -- is carved from one fixed
+ * 128 MiB array per device (scrna_noalloc_scratch_bytes), which each device
+ * the library runs on holds until the process ends.  A run that does not fit
+ * returns SCRNA_ERR_INVALID (roughly cells x G x 9 bytes of state and
+ * promoters, plus 4 per captured count and 8 per snapshot element).  Results
+ * are bit-identical to scrna_simulate_cle.  All calls on a device share the
+ * array, which keeps the last call's contents; calls that overlapped on one
+ * device would race on it, so the library serialises them per device.
+ * Outputs are host buffers, as for scrna_simulate_cle. */
+int scrna_simulate_cle_noalloc(scrna_ctx* ctx, const scrna_network* net, const scrna_kinetics* kin,
+                               const scrna_run* run, const scrna_capture* capture, scrna_output* out);
+/* Size in bytes of the fixed per-device array of scrna_simulate_cle_noalloc. */
+uint64_t scrna_noalloc_scratch_bytes(void);
+
 /* Replaces SyntheticscRNAseq.simulate_gpu_batch(networks, ::CLE, kin;
  * cell_num, T, readout)  (ext:132-213).  Unlike the reference, networks may
  * differ in G, cell_num, kinetics and horizon: runs[i] belongs to nets[i]. */

@@ -22,7 +22,7 @@ OBJDIR = os.path.join(HERE, "_build")
 
 SOURCES = ["cle_kernels.hip", "runtime.cpp", "population.cpp", "server.cpp", "dense_kernels.hip",
            "autotune.cpp", "unfused.hip", "tauleap_kernels.hip", "prior.cpp",
-           "mtx_writer.cpp"]
+           "mtx_writer.cpp", "noalloc.cpp"]
 
 COMMON = [
     "--offload-arch=gfx950",

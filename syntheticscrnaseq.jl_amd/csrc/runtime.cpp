@@ -604,6 +604,18 @@ void plan_release(scrna_plan* p) {
 }
 
 int plan_alloc(scrna_plan* p, void** ptr, size_t bytes) {
+    if (p->arena) {
+        Arena& a = *p->arena;
+        const size_t off = (a.used + 255) & ~(size_t)255;
+        if (off > a.size || bytes > a.size - off)
+            return set_error(SCRNA_ERR_INVALID,
+                             "run needs more than the %zu-byte fixed scratch of the allocation-free entry point",
+                             a.size);
+        *ptr = bytes ? a.base + off : nullptr;
+        a.used = off + bytes;
+        p->bytes += bytes;
+        return SCRNA_OK;
+    }
     int rc = ctx_alloc(p->ctx, ptr, bytes);
     if (rc) return rc;
     if (*ptr) p->allocs.push_back({*ptr, bytes});
@@ -613,7 +625,7 @@ int plan_alloc(scrna_plan* p, void** ptr, size_t bytes) {
 
 int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna_kinetics* kins,
                const scrna_run* runs, const scrna_population* pops, const scrna_capture* caps,
-               int force_ch, scrna_plan** out) {
+               int force_ch, scrna_plan** out, Arena* arena) {
     if (!ctx || !out || n <= 0 || !nets || !kins || !runs)
         return set_error(SCRNA_ERR_INVALID, "plan needs a context, n > 0 and network/kinetics/run arrays");
     for (int i = 0; i < n; ++i) {
@@ -622,6 +634,15 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
     }
     std::unique_ptr<scrna_plan> P(new scrna_plan());
     P->ctx = ctx;
+    P->arena = arena;
+    if (arena) {
+        // Arena plans run on one stream with no events, and hold no
+        // population buffers or library handles of their own.
+        for (int i = 0; i < n; ++i)
+            if ((pops && pops[i].enabled) || runs[i].precision == SCRNA_PREC_FP32_UNFUSED)
+                return set_error(SCRNA_ERR_UNSUPPORTED,
+                                 "population runs and the reference-structured FP32 path allocate their own buffers");
+    }
     P->nets.resize(n);
     std::vector<uint32_t> blobs;
     size_t state_elems = 0, snap_elems = 0, prom_elems = 0, vol_elems = 0, inj_elems = 0, count_elems = 0,
@@ -857,8 +878,9 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
     PLAN_ALLOC(d_sched, int, P->sched_words);
 #undef PLAN_ALLOC
     // Every launch clears its block with a kernel first; clearing it here as
-    // well means no error word is ever read uninitialised.
-    if (P->d_sched && hipMemset(P->d_sched, 0, P->sched_words * sizeof(int)) != hipSuccess) {
+    // well means no error word is ever read uninitialised.  (Arena plans read
+    // only the blocks their own launch cleared.)
+    if (P->d_sched && !arena && hipMemset(P->d_sched, 0, P->sched_words * sizeof(int)) != hipSuccess) {
         plan_release(P.get());
         return set_error(SCRNA_ERR_DEVICE, "hipMemset of the schedule block failed");
     }
@@ -910,6 +932,13 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
         plan_release(P.get());
         return set_error(SCRNA_ERR_DEVICE, "upload sync failed: %s", hipGetErrorString(e));
     }
+    for (auto& cn : P->nets) P->cell_gene_steps += (double)cn.G * (double)cn.cells * (double)cn.n_steps;
+    if (arena) {
+        for (auto& g : P->groups) g.lane = 0;
+        P->n_lanes = 1;
+        *out = P.release();
+        return SCRNA_OK;
+    }
     if (hipEventCreate(&P->ev_start) != hipSuccess || hipEventCreate(&P->ev_stop) != hipSuccess ||
         hipEventCreateWithFlags(&P->ev_done, hipEventDisableTiming) != hipSuccess) {
         plan_release(P.get());
@@ -943,7 +972,6 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
             return set_error(SCRNA_ERR_DEVICE, "stream or event creation failed");
         }
     }
-    for (auto& cn : P->nets) P->cell_gene_steps += (double)cn.G * (double)cn.cells * (double)cn.n_steps;
     *out = P.release();
     return SCRNA_OK;
 }
@@ -1013,7 +1041,7 @@ int launch_steps(scrna_plan* P, hipStream_t st, int s_begin, int s_end, bool seg
 }
 
 // Every kernel of one simulation, in stream order (also the body of the graph).
-static int plan_enqueue(scrna_plan* P, hipStream_t st, bool events) {
+int plan_enqueue(scrna_plan* P, hipStream_t st, bool events) {
     if (events) HIP_TRY(hipEventRecord(P->ev_start, st));
     if (P->has_population) {
         int rc = population_launch(P, st);
@@ -1045,6 +1073,17 @@ int plan_launch(scrna_plan* P, hipStream_t st) {
     }
     HIP_TRY(hipEventRecord(P->ev_done, st));
     P->launched = true;
+    return SCRNA_OK;
+}
+
+// After a finished launch: the error word of every segmented launch group.
+int plan_check_schedule(scrna_plan* P) {
+    for (const auto& lg : P->groups) {
+        if (lg.kind == 1 || lg.nseg <= 1) continue;
+        int err = 0;
+        HIP_TRY(hipMemcpy(&err, P->d_sched + lg.sched_off + SCHED_ERROR, sizeof(int), hipMemcpyDeviceToHost));
+        if (err) return set_error(SCRNA_ERR_DEVICE, "stepper segment waited past its time bound");
+    }
     return SCRNA_OK;
 }
 
@@ -1175,6 +1214,7 @@ int scrna_ctx_create(int32_t device, uint64_t hbm_budget_bytes, scrna_ctx** out)
     }
     C->lds_limit_soft = std::min(C->lds_limit_soft, C->lds_max);
     HIP_TRY(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
+    noalloc_scratch_init(device);   // this device's address of the allocation-free entry point's array
     *out = C.release();
     return SCRNA_OK;
 }
@@ -1212,13 +1252,7 @@ int scrna_plan_sync(scrna_plan* plan) {
     // before the moments and capture kernels): other work on the device is
     // not waited for.
     HIP_TRY(hipEventSynchronize(plan->ev_done));
-    for (const auto& lg : plan->groups) {
-        if (lg.kind == 1 || lg.nseg <= 1) continue;
-        int err = 0;
-        HIP_TRY(hipMemcpy(&err, plan->d_sched + lg.sched_off + SCHED_ERROR, sizeof(int), hipMemcpyDeviceToHost));
-        if (err) return set_error(SCRNA_ERR_DEVICE, "stepper segment waited past its time bound");
-    }
-    return SCRNA_OK;
+    return plan_check_schedule(plan);
 }
 
 int scrna_plan_download(scrna_plan* plan, int32_t i, scrna_output* out) {

@@ -82,8 +82,19 @@ struct LaunchGroup {
 
 }  // namespace scrna
 
+namespace scrna {
+// A fixed device region that a plan carves its buffers from instead of
+// allocating (bump order, 256-byte aligned).  Used by the allocation-free
+// entry point; such a plan owns no memory, events or streams.
+struct Arena {
+    char* base = nullptr;
+    size_t size = 0, used = 0;
+};
+}  // namespace scrna
+
 struct scrna_plan {
     scrna_ctx* ctx = nullptr;
+    scrna::Arena* arena = nullptr;        // non-null: buffers come from this region
     std::vector<scrna::CompiledNet> nets;
     std::vector<scrna::LaunchGroup> groups;
     std::vector<scrna::UnfusedNet> unfused;
@@ -142,8 +153,11 @@ int compile_network(const scrna_network* net, const scrna_kinetics* kin, const s
                     const scrna_population* pop, int force_ch, CompiledNet* out);
 int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna_kinetics* kins,
                const scrna_run* runs, const scrna_population* pops, const scrna_capture* caps, int force_ch,
-               scrna_plan** out);
+               scrna_plan** out, Arena* arena = nullptr);
 int plan_launch(scrna_plan* P, hipStream_t st);
+int plan_enqueue(scrna_plan* P, hipStream_t st, bool events);
+int plan_check_schedule(scrna_plan* P);
+void noalloc_scratch_init(int device);
 int plan_download(scrna_plan* P, int32_t i, scrna_output* out);
 void plan_release(scrna_plan* P);
 int population_launch(scrna_plan* P, hipStream_t st);

@@ -154,6 +154,9 @@ SIGNATURES = {
     "scrna_simulate_cle": (C.c_int, [_VP, C.POINTER(ScrnaNetwork), C.POINTER(ScrnaKinetics),
                                      C.POINTER(ScrnaRun), C.POINTER(ScrnaPopulation),
                                      C.POINTER(ScrnaCapture), C.POINTER(ScrnaOutput)]),
+    "scrna_simulate_cle_noalloc": (C.c_int, [_VP, C.POINTER(ScrnaNetwork), C.POINTER(ScrnaKinetics),
+                                             C.POINTER(ScrnaRun), C.POINTER(ScrnaCapture), C.POINTER(ScrnaOutput)]),
+    "scrna_noalloc_scratch_bytes": (C.c_uint64, []),
     "scrna_simulate_cle_batch": (C.c_int, [_VP, C.c_int32, C.POINTER(ScrnaNetwork),
                                            C.POINTER(ScrnaKinetics), C.POINTER(ScrnaRun),
                                            C.POINTER(ScrnaPopulation), C.POINTER(ScrnaCapture),
