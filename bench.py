@@ -1,21 +1,32 @@
 """Benchmark: GPU CLE throughput in cell*gene*steps/s on one MI355X.
 
-Default workload (one timed "step" = one full simulate() pass): a seeded
-G=20 mixed-regulation network with telegraph promoters on every gene,
-100,000 cells, 5000 Euler-Maruyama steps at dt=0.01, a snapshot every 500
-steps, FP64 moment sums per snapshot and LogNormal-Binomial capture on the
-final state.  Inputs (the compiled network) are resident in HBM before the
-timed region; outputs stay in HBM.  Prints one JSON line with
-  roofline      VALU issue roofline of the stepper kernel (the dominant one),
-                timed with HIP events on its launch stream;
-  cpu_baseline  the FP64 CPU restatement (oracle/) timed on this host's cores
-                on a bounded sample of the same workload.
+Default (--workload headline): the metric "BF16-MFMA vs FP32 CLE at <=1%
+moment error; p99 latency", measured in one run.
+  * value: the G=256 precision workload (additive network with 25% dense A,
+    8 AND + 8 OR edges, 65,536 cells, 2000 Euler-Maruyama steps at dt=0.005)
+    with the regulation contraction in BF16 on MFMA and FP32 state; one timed
+    "step" = one full simulate() pass, inputs resident in HBM.
+  * precision: the same workload with BF16x3 split precision, the fused
+    FP32 forms and the reference-structured FP32 form (one kernel per
+    broadcast, two SGEMMs per step), their speedups, and per-gene mean and
+    variance errors against an FP64 CPU replay of a cell subset drawing the
+    identical Philox stream.
+  * g20: the G=20 telegraph workload (100,000 cells, 5000 steps, snapshots
+    every 500, capture), the FP32 stepper's throughput and roofline.
+  * serving: Poisson arrivals at a fixed QPS, p50 / p99 latency against the
+    stated bound, aggregate throughput.
+  * roofline: the dominant kernel of `value` (the dense MFMA stepper), timed
+    with HIP events on its launch stream.
+  * cpu_baseline / cpu_baselines: CPU restatements of the reference CLE
+    (oracle/cpu_ref.py: cle.jl's cell-vectorised form with OpenBLAS products)
+    and CLEFast's per-cell form (the FP64 oracle, OpenMP over cells), at one
+    thread and at all cores used, on the G=5 workload, a G=20 slice and a G=256
+    slice, with the host CPU named.
 
-Other workloads (--workload): g256 (precision comparison, 65,536 cells, 2000
-steps), batch (4096 uneven networks with population dynamics in one hipGraph),
-serve (Poisson arrivals at a fixed QPS), tauleap (binomial tau-leap on the G=20
-network), sbi (networks drawn from the prior, simulated in one plan), copy
-(plain device copy of known size, for scaling hardware byte counters).
+Other workloads (--workload): g20, g256, batch (4096 uneven networks with
+population dynamics in one hipGraph), serve, tauleap (binomial tau-leap on the
+G=20 network), sbi (networks drawn from the prior, simulated in one plan),
+copy (plain device copy of known size, for scaling hardware byte counters).
 """
 from __future__ import annotations
 
@@ -95,29 +106,109 @@ def valu_census(net, dense: bool = False) -> dict:
             "fp32_equiv_flops": 2.0 * fma_eq}
 
 
-def cpu_baseline(wl, target_s: float = 12.0) -> dict:
-    """Time the FP64 oracle (a C restatement of cle.jl) on a bounded cell sample."""
+def host_cpu() -> dict:
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count() or 1}
+
+
+def _threads_all() -> int:
+    # the GPU box gives one GPU a 16-core share whatever os.cpu_count() reports
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def _sized(run, n_max: int, probe: int, target_s: float, quantum: int = 1):
+    """Time run(probe), then run(n) with n (cells or steps) sized to about target_s seconds."""
+    t0 = time.perf_counter()
+    run(probe)
+    rate = probe / max(time.perf_counter() - t0, 1e-6)
+    n = int(min(n_max, max(probe, rate * target_s)))
+    n = max(quantum, (n // quantum) * quantum)
+    t0 = time.perf_counter()
+    run(n)
+    return n, time.perf_counter() - t0
+
+
+def time_restatement(name: str, case, threads: int, target_s: float) -> dict:
+    """Time one CPU restatement on a cell sample of one workload case
+    (label, net, kinetics, dt, steps, cells of the workload, init)."""
+    from oracle import cpu_ref
     from oracle import oracle as orc
 
-    threads = max(1, min(16, os.cpu_count() or 1))
-    net, kin = wl["net"], wl["kin"]
-    steps = int(np.ceil(wl["T"] / wl["dt"]))
-    probe_cells = 16 * threads
-    t0 = time.perf_counter()
-    orc.cle(net, kin, dt=wl["dt"], n_steps=steps, cell_num=wl["cells"], cell_lo=0, cell_hi=probe_cells,
-            seed=1, threads=threads)
-    dt_probe = time.perf_counter() - t0
-    rate = probe_cells / max(dt_probe, 1e-6)
-    cells = int(min(wl["cells"], max(probe_cells, rate * target_s)))
-    cells = max(threads, (cells // threads) * threads)
-    t0 = time.perf_counter()
-    orc.cle(net, kin, dt=wl["dt"], n_steps=steps, cell_num=wl["cells"], cell_lo=0, cell_hi=cells, seed=1,
-            threads=threads)
-    el = time.perf_counter() - t0
+    label, net, k, dt, steps, cells_max, init = case
+    if name == "cle_ref":
+        # cell-vectorised: up to 16384 of the workload's cells at once (its per-step cost is linear
+        # in cells from there on), a sample of its steps
+        cells = min(cells_max, 16384)
+
+        def run(n):
+            cpu_ref.cle_ref(net, k, dt=dt, n_steps=n, cells=cells, seed=1, init=init, threads=threads, noise="randn")
+        steps, el = _sized(run, steps, 4, target_s)
+    else:
+        # one cell at a time: a sample of the cells, the whole horizon
+        def run(c):
+            orc.cle(net, k, dt=dt, n_steps=steps, cell_num=cells_max, cell_lo=0, cell_hi=c, seed=1, init=init,
+                    threads=threads)
+        cells, el = _sized(run, cells_max, 4 * threads, target_s, quantum=threads)
     cgs = float(cells) * net.G * steps
-    return {"value": cgs / el, "unit": "cell*gene*steps/s", "cores": threads, "kind": "port",
-            "sample": f"{cells} of {wl['cells']} cells x G={net.G} x {steps} steps (full horizon), "
-                      f"FP64 C restatement of cle.jl with telegraph + AND/OR, OpenMP over cells, {el:.1f} s"}
+    log(f"cpu {name} {label[:12]} x{threads}: {cgs / el:.3e} c*g*s/s ({cells} cells, {el:.1f} s)")
+    return {"workload": label, "restatement": name, "threads": threads, "cells": cells, "steps": steps,
+            "seconds": el, "value": cgs / el, "unit": "cell*gene*steps/s"}
+
+
+def cpu_baselines(target_s: float = 4.0) -> tuple:
+    """CPU restatements of the reference CPU paths, timed on this host.
+
+    cle_ref (oracle/cpu_ref.py): cle.jl:34-130's cell-vectorised structure,
+      regulation as two OpenBLAS products per step, fused elementwise passes,
+      two ziggurat noise matrices per step (randn!); its threads are the BLAS
+      threads (the elementwise passes run on one, as Julia's broadcasts do).
+    clefast_ref (oracle/, orc_cle_simulate): cle_optimized.jl:48-152's
+      structure, one cell at a time with a dense G x G product, OpenMP over
+      cells (CLEFast's Polyester threads), FP64.
+    Each runs at 1 thread and at all threads used, on the G=5 workload (5000
+    cells from zero counts, 5000 steps; full horizon, cell sample), a G=20
+    telegraph slice and a G=256 slice: cle_ref runs up to 16384 cells of the
+    workload at once over a sample of its steps, clefast_ref a sample of the
+    cells over the whole horizon (at most 400 steps at G=256); c*g*s/s is the
+    same for any such slice of a run whose per-step cost is constant.  Returns (primary, entries): the
+    primary baseline is cle_ref at all threads on the G=256 slice, the
+    workload of `value`."""
+    from scrna_amd import samplers
+
+    T_all = _threads_all()
+    w0 = samplers.cpu_g5_workload()
+    w1 = samplers.gpu_g20_workload()
+    g256 = samplers.dense_g256_network()
+    kin = samplers.WORKLOAD_KINETICS
+    cases = [("G=5 workload (5000 cells, 5000 steps, zero init)", w0["net"], w0["kin"], w0["dt"], 5000, 5000, "zeros"),
+             ("G=20 telegraph workload slice (5000 steps)", w1["net"], w1["kin"], w1["dt"], 5000, 100_000, "steady"),
+             ("G=256 precision workload slice (400 of 2000 steps, dt=0.005)", g256, kin, 0.005, 400, 65536, "steady")]
+    entries = []
+    primary = None
+    for case in cases:
+        label, net = case[0], case[1]
+        for name in ("cle_ref", "clefast_ref"):
+            if name == "clefast_ref" and net.G > 64:
+                continue          # CLEFast specialises on G (StaticArrays); the reference has no G=256 form of it
+            for threads in (1, T_all):
+                e = time_restatement(name, case, threads, target_s)
+                entries.append(e)
+                cells, steps, el = e["cells"], e["steps"], e["seconds"]
+                cgs = float(cells) * net.G * steps
+                if name == "cle_ref" and net.G == 256 and threads == T_all:
+                    primary = {"value": cgs / el, "unit": "cell*gene*steps/s", "cores": threads, "kind": "port",
+                               "sample": f"{cells} of 65536 cells x G=256 x {steps} of the 2000 steps of the precision "
+                                         f"workload, {el:.1f} s: cle.jl's cell-vectorised FP64 form (OpenBLAS products "
+                                         f"on {threads} threads, fused elementwise passes, ziggurat noise), host "
+                                         f"{host_cpu()['model']}"}
+    return primary, entries
 
 
 def timed_runs(plan, steps: int, warmup: int):
@@ -133,6 +224,29 @@ def timed_runs(plan, steps: int, warmup: int):
         kernel_ms.append(ms)
     plan.sync()
     return time.perf_counter() - t0, kernel_ms
+
+
+def source_digest(files) -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for f in files:
+        h.update(open(os.path.join(ROOT, "syntheticscrnaseq.jl_amd", "csrc", f), "rb").read())
+    return h.hexdigest()[:16]
+
+
+def measured_traffic(name: str, sources, applies: bool):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes (profiles/<name>, copy-calibrated).  null when the file is
+    absent, was measured on other kernel sources than the ones built now, or
+    the workload differs from the profiled one."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not applies or not os.path.exists(path):
+        return None
+    rec = json.load(open(path))
+    if rec.get("source_digest") != source_digest(sources):
+        log(f"{name}: measured on other kernel sources; traffic reported as null")
+        return None
+    return rec["traffic_bytes"]
 
 
 def run_g20(a, sc) -> dict:
@@ -154,12 +268,8 @@ def run_g20(a, sc) -> dict:
     k_ms = float(np.mean(kernel_ms))
     census = valu_census(net)
     achieved = census["fp32_equiv_flops"] * cgs_per_run / (k_ms * 1e-3) / 1e12
-    # HBM bytes per stepper launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
-    # workload (copy-calibrated); null when that measurement is absent or for another cell count.
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "cle_stepper_g20_traffic.json")
-    if os.path.exists(tpath) and wl["cells"] == 100_000:
-        traffic = json.load(open(tpath))["traffic_bytes"]
+    traffic = measured_traffic("cle_stepper_g20_traffic.json", ("cle_kernels.hip", "device_rng.h", "blob.h"),
+                               wl["cells"] == 100_000)
     out = {
         "metric": "cell*gene*steps/s (GPU CLE, FP32 state)",
         "value": value,
@@ -186,9 +296,15 @@ def run_g20(a, sc) -> dict:
     }
     if a.tile:
         out["config"]["genes_per_lane"] = a.tile
-    if not a.no_cpu:
+    if not a.no_cpu and a.workload == "g20":
         log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(wl)
+        e = time_restatement("clefast_ref", ("G=20 telegraph workload", net, kin, wl["dt"], steps_per_run,
+                                             wl["cells"], wl["init"]), _threads_all(), 10.0)
+        out["cpu_baseline"] = {"value": e["value"], "unit": e["unit"], "cores": e["threads"], "kind": "port",
+                               "sample": f"{e['cells']} of {wl['cells']} cells x G=20 x {steps_per_run} steps (full "
+                                         f"horizon), {e['seconds']:.1f} s: the FP64 per-cell restatement (CLEFast's "
+                                         f"structure) with telegraph + AND/OR, OpenMP over cells, host "
+                                         f"{host_cpu()['model']}"}
     plan.close()
     return out
 
@@ -211,11 +327,13 @@ def run_g256(a, sc) -> dict:
     precs = [p for p in ("fp32_unfused", "fp32", "bf16", "bf16x3") if p in a.precisions.split(",")]
     for prec in precs:
         plan = sc.Plan([net], sc.CLE(dt), kin, cell_num=cells, T=T, readout="both", seed=777, precision=prec)
-        el, kms = timed_runs(plan, a.steps, a.warmup)
+        # the timed forms of `value` run the requested count; comparison forms at most 5 (they take up to 1.3 s each)
+        n_runs = a.steps if prec == a.headline_precision else min(a.steps, 5)
+        el, kms = timed_runs(plan, n_runs, min(a.warmup, 2))
         cgs = float(net.G) * cells * steps
         k_ms = float(np.mean(kms))
         census = valu_census(net, dense=prec in ("bf16", "bf16x3"))
-        r = {"cgs_per_s": cgs * a.steps / el, "kernel_ms": k_ms,
+        r = {"cgs_per_s": cgs * n_runs / el, "kernel_ms": k_ms, "runs": n_runs, "ms_per_run": el * 1e3 / n_runs,
              "valu_frac": census["fp32_equiv_flops"] * cgs / (k_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
              "census_per_cgs": census}
         if prec in ("bf16", "bf16x3"):
@@ -242,6 +360,7 @@ def run_g256(a, sc) -> dict:
                 errs[("mrna", "protein")[k]] = {"max_mean_rel_err": float(me.max()), "max_var_rel_err": float(ve.max())}
             out["results"][prec]["vs_fp64_replay"] = errs
         out["replay_cells"] = n_sub
+    out["fp32_fused_form"] = "sparse FP32 VALU stepper (fp32): 16 genes per lane, in-edges streamed per step"
     res = out["results"]
     for prec in ("fp32", "bf16", "bf16x3"):
         if prec not in res:
@@ -250,6 +369,71 @@ def run_g256(a, sc) -> dict:
             res[prec]["speedup_vs_fp32_fused"] = res[prec]["cgs_per_s"] / res["fp32"]["cgs_per_s"]
         if "fp32_unfused" in res:
             res[prec]["speedup_vs_fp32_reference_structured"] = res[prec]["cgs_per_s"] / res["fp32_unfused"]["cgs_per_s"]
+    return out
+
+
+def run_headline(a, sc) -> dict:
+    """The metric line: value = BF16-MFMA throughput at G=256 with the precision
+    comparison, the G=20 workload and the serving workload as fields."""
+    from scrna_amd import samplers
+
+    a.headline_precision = "bf16"
+    g = run_g256(a, sc)
+    res = g["results"]
+    bf = res["bf16"]
+    net = samplers.dense_g256_network()
+    cgs = float(net.G) * g["cells"] * g["steps"]
+    census = bf["census_per_cgs"]
+    achieved = census["fp32_equiv_flops"] * cgs / (bf["kernel_ms"] * 1e-3) / 1e12
+    traffic = measured_traffic("cle_dense_g256_traffic.json", ("dense_kernels.hip", "device_rng.h", "blob.h"),
+                               g["cells"] == 65536)
+    errs = bf.get("vs_fp64_replay", {})
+    worst = max([v for sp in errs.values() for v in sp.values()] or [float("nan")])
+    log("g20 workload")
+    g20 = run_g20(a, sc)
+    log("serving workload")
+    srv = run_serve(a, sc)
+    out = {
+        "metric": "cell*gene*steps/s, BF16-MFMA vs FP32 CLE at <=1% moment err; p99 latency",
+        "value": bf["cgs_per_s"],
+        "unit": "cell*gene*steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": bf["ms_per_run"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic (seeded G=256 network, Philox noise)",
+        "config": {"workload": "G=256 additive network, 25% dense A (+-U(0.5,3)), 8 AND + 8 OR, K=10 n=2, constitutive, "
+                               "65536 cells, 2000 steps dt=0.005; regulation contraction in BF16 on MFMA with FP32 "
+                               "accumulation, FP32 state",
+                   "G": net.G, "cells": g["cells"], "sim_steps": g["steps"], "parallelism": "single GPU"},
+        "headline": {
+            "bf16_vs_fp32_fused": bf.get("speedup_vs_fp32_fused"),
+            "bf16_vs_fp32_reference_structured": bf.get("speedup_vs_fp32_reference_structured"),
+            "bf16_max_moment_rel_err_vs_fp64": worst,
+            "bf16_within_1pct": bool(worst <= 0.01),
+            "fp32_fused_form": g.get("fp32_fused_form"),
+            "serving_qps": srv["qps"], "serving_p50_ms": srv["p50_ms"], "serving_p99_ms": srv["p99_ms"],
+            "serving_p99_bound_ms": srv["p99_bound_ms"], "serving_p99_within_bound": srv["p99_within_bound"],
+        },
+        "precision": g,
+        "g20": {k: g20[k] for k in ("value", "ms_per_step", "config", "roofline")},
+        "serving": srv,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s (FP32-FMA-equivalent VALU issue)", "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": traffic, "kernel": "cle_dense_kernel (BF16 MFMA)", "kernel_ms": bf["kernel_ms"],
+                     "census_per_cgs": census,
+                     "mfma_tflops": bf.get("mfma_tflops_issued"),
+                     "mfma_frac": (bf.get("mfma_tflops_issued") or 0.0) / BF16_PEAK_TFLOPS},
+    }
+    if not a.no_cpu:
+        log("cpu baselines")
+        primary, entries = cpu_baselines()
+        out["cpu_baseline"] = primary
+        out["cpu_baselines"] = {"host": host_cpu(), "threads_all": _threads_all(), "runs": entries}
     return out
 
 
@@ -423,8 +607,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["g20", "g256", "batch", "serve", "copy", "tauleap", "sbi"],
-                    default="g20")
+    ap.add_argument("--workload", choices=["headline", "g20", "g256", "batch", "serve", "copy", "tauleap", "sbi"],
+                    default="headline")
     ap.add_argument("--cells", type=int, default=None, help="override cell count (testing only)")
     ap.add_argument("--tile", type=int, default=0, help="genes per lane for the FP32 stepper (2, 4, 8, 16)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -449,7 +633,8 @@ def main() -> int:
 
     if a.tile:
         sc.default_context().set_tile(1, min(256, 16 * a.tile), a.tile)
-    runner = {"g20": run_g20, "g256": run_g256, "batch": run_batch, "serve": run_serve,
+    a.headline_precision = None
+    runner = {"headline": run_headline, "g20": run_g20, "g256": run_g256, "batch": run_batch, "serve": run_serve,
               "tauleap": run_tauleap, "sbi": run_sbi}[a.workload]
     print(json.dumps(runner(a, sc)))
     return 0

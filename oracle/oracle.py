@@ -80,6 +80,10 @@ def load():
         lib.orc_divide.argtypes = [C.POINTER(OrcPopulation), C.c_int32, C.c_int64, C.c_uint64, C.c_uint32,
                                    C.c_int64, _dp, _dp, _dp, _lp, _lp, _dp, _dp, _dp, _dp, _lp]
         lib.orc_divide.restype = C.c_int
+        lib.orc_ref_hill.argtypes = [_dp, C.c_int64, C.c_double, C.c_double, _dp, _dp]
+        lib.orc_ref_init.argtypes = [C.POINTER(OrcNetwork), C.POINTER(OrcKinetics), C.POINTER(OrcRun), _dp, _dp, _up]
+        lib.orc_ref_update.argtypes = [C.POINTER(OrcNetwork), C.POINTER(OrcKinetics), C.POINTER(OrcRun), C.c_int64,
+                                       _dp, _dp, _dp, _dp, _up, _dp, _dp]
         lib.orc_tauleap_simulate.argtypes = lib.orc_cle_simulate.argtypes
         lib.orc_tauleap_simulate.restype = C.c_int
         lib.orc_tauleap_population.argtypes = lib.orc_cle_population.argtypes

@@ -372,6 +372,104 @@ int orc_cle_simulate(const orc_network* net, const orc_kinetics* kin, const orc_
 }
 
 /* ---------------------------------------------------------------------- */
+/* Cell-vectorised CLE pieces for the CPU baseline restatement of          */
+/* cle.jl:34-130 (oracle/cpu_ref.py runs the step; between orc_ref_hill and */
+/* orc_ref_update it forms the regulation with two OpenBLAS products, as   */
+/* cle.jl:84-86).  Arrays are Julia's G x cells column-major state: element */
+/* (g, c) at c*G + g.  Draws come from the same Philox stream as           */
+/* orc_cle_simulate, so the restatement reproduces the oracle's runs.      */
+/* ---------------------------------------------------------------------- */
+void orc_ref_hill(const double* P, int64_t n, double hill_n, double Kn, double* act, double* rep) {
+    for (int64_t i = 0; i < n; ++i) {
+        const double pn = pow(P[i], hill_n);  /* cle.jl:79 */
+        const double den = pn + Kn;           /* cle.jl:80 */
+        act[i] = pn / den;                    /* cle.jl:81 */
+        rep[i] = Kn / den;                    /* cle.jl:82 */
+    }
+}
+
+void orc_ref_init(const orc_network* net, const orc_kinetics* kin, const orc_run* run, double* M, double* P,
+                  uint8_t* on) {
+    const int G = net->G;
+    orc_tele tele;
+    orc_tele_init(net, run->dt, &tele);
+    for (int64_t c = 0; c < run->cell_num; ++c) {
+        for (int g = 0; g < G; ++g) {
+            const double mss = net->basals[g] / (kin->mu_m + kin->dilution);   /* kinetics.jl:65-70 */
+            const double pss = kin->k_t * mss / (kin->mu_p + kin->dilution);
+            M[c * G + g] = run->init_mode == 0 ? ceil(mss) : 0.0;             /* cle.jl:52-53 */
+            P[c * G + g] = run->init_mode == 0 ? ceil(pss) : 0.0;
+        }
+        orc_promoter_init(&tele, G, run->seed, run->net_id << 4, (uint32_t)c, on + c * G);
+    }
+    orc_tele_free(&tele);
+}
+
+void orc_ref_update(const orc_network* net, const orc_kinetics* kin, const orc_run* run, int64_t step, double* M,
+                    double* P, double* reg, const double* act, uint8_t* on, const double* noise_m,
+                    const double* noise_p) {
+    const int G = net->G;
+    const int64_t C = run->cell_num;
+    const double dt = run->dt, sqrt_dt = sqrt(dt);
+    const double k_t = kin->k_t, mu_m = kin->mu_m + kin->dilution, mu_p = kin->mu_p + kin->dilution;
+    const uint32_t c3base = run->net_id << 4;
+    orc_tele tele;
+    orc_tele_init(net, dt, &tele);
+    for (int64_t c = 0; c < C; ++c) {
+        double* R = reg + c * G;
+        const double* a = act + c * G;
+        for (int e = 0; e < net->n_coop; ++e) {            /* cle.jl:91-97 */
+            double prod = 1.0;
+            for (int k = net->coop_off[e]; k < net->coop_off[e + 1]; ++k) prod *= a[net->coop_src[k]];
+            const int t = net->coop_target[e];
+            R[t] += net->coop_strength[e] * net->basals[t] * prod;
+        }
+        for (int e = 0; e < net->n_redun; ++e) {           /* cle.jl:98-104 */
+            double prod = 1.0;
+            for (int k = net->redun_off[e]; k < net->redun_off[e + 1]; ++k) prod *= (1.0 - a[net->redun_src[k]]);
+            const int t = net->redun_target[e];
+            R[t] += net->redun_strength[e] * net->basals[t] * (1.0 - prod);
+        }
+    }
+    for (int64_t c = 0; c < C; ++c) {
+        uint32_t u[4] = {0, 0, 0, 0};
+        for (int g = 0; g < G; ++g) {
+            const int64_t i = c * G + g;
+            const int o = (g & 1) * 2;
+            double xm, xp;
+            if ((g & 1) == 0 && ((!run->injected && !noise_m) || tele.telegraph_any))
+                orc_draw(run->seed, (uint32_t)c, (uint32_t)step, (uint32_t)(g >> 1), c3base | ORC_P_NORMAL, u);
+            if (tele.telegraph_any && tele.switching[g] &&
+                orc_cle_flip(u[o], u[o + 1], on[i] ? tele.thr_off[g] : tele.thr_on[g], run->seed, (uint32_t)c,
+                             (uint32_t)step, g, c3base))
+                on[i] ^= 1;
+            if (noise_m) {                             /* this step's noise matrices (cle.jl:109, 116) */
+                xm = noise_m[i];
+                xp = noise_p[i];
+            } else if (run->injected) {
+                const size_t base = ((size_t)step * G + g) * 2;
+                xm = run->injected[(base + 0) * (size_t)C + c];
+                xp = run->injected[(base + 1) * (size_t)C + c];
+            } else {
+                orc_normal_pair(u[o], u[o + 1], &xm, &xp);
+            }
+            const double tx = on[i] ? (reg[i] + net->basals[g]) : 0.0;
+            const double mm = M[i];
+            double var_m = tx + mu_m * (mm > 0.0 ? mm : 0.0);                   /* cle.jl:111 */
+            if (var_m < 0.0) var_m = 0.0;
+            const double mn = mm + (tx - mu_m * mm) * dt + xm * sqrt_dt * sqrt(var_m);   /* cle.jl:112 */
+            M[i] = mn > 0.0 ? mn : 0.0;
+            const double pp = P[i];
+            double var_p = k_t * M[i] + mu_p * (pp > 0.0 ? pp : 0.0);           /* cle.jl:118 */
+            if (var_p < 0.0) var_p = 0.0;
+            const double pn = pp + (k_t * M[i] - mu_p * pp) * dt + xp * sqrt_dt * sqrt(var_p);   /* cle.jl:119 */
+            P[i] = pn > 0.0 ? pn : 0.0;
+        }
+    }
+    orc_tele_free(&tele);
+}
+
+/* ---------------------------------------------------------------------- */
 /* Binomial sampler: inversion (small n*p) and BTRD (Hormann 1993).        */
 /* Uniforms come from consecutive Philox blocks, two 52-bit uniforms each. */
 /* ---------------------------------------------------------------------- */

@@ -138,6 +138,21 @@ int orc_divide(const orc_population* pop, int32_t G, int64_t N, uint64_t seed, u
                double* m, double* p, double* V, int64_t* n_div, int64_t* dividers, double* mother_m,
                double* mother_p, double* daughter_m, double* daughter_p, int64_t* replaced);
 
+/* Cell-vectorised CLE pieces for the CPU baseline restatement of cle.jl:34-130
+ * (state G x cell_num column-major, element (g, c) at c*G + g; all
+ * cell_num cells).  orc_ref_hill: act, rep of cle.jl:79-82.  orc_ref_init:
+ * initial state and promoters.  orc_ref_update: given reg = A_pos*act +
+ * A_neg*rep, the AND/OR corrections (cle.jl:89-106), telegraph switching and
+ * the mRNA then protein update of step `step` (cle.jl:108-120), with the
+ * draws of orc_cle_simulate, or with noise_m / noise_p (same layout as M,
+ * both or neither) as this step's two noise matrices. */
+void orc_ref_hill(const double* P, int64_t n, double hill_n, double Kn, double* act, double* rep);
+void orc_ref_init(const orc_network* net, const orc_kinetics* kin, const orc_run* run, double* M, double* P,
+                  uint8_t* on);
+void orc_ref_update(const orc_network* net, const orc_kinetics* kin, const orc_run* run, int64_t step, double* M,
+                    double* P, double* reg, const double* act, uint8_t* on, const double* noise_m,
+                    const double* noise_p);
+
 /* Binomial tau-leap of cells [cell_lo, cell_hi) (tauleap_binomial.jl:31-180):
  * same layouts as orc_cle_simulate; counts are returned as doubles.  init_mode
  * 0 rounds the steady state half to even (tauleap_binomial.jl:61-62).
