@@ -8,6 +8,12 @@
 // MODE 1 (BF16): act and W rounded to bf16.
 // MODE 2 (BF16X3): act = a_hi + a_lo, W = W_hi + W_lo (bf16 each) and
 //   reg = a_hi W_hi + a_lo W_hi + a_hi W_lo, recovering ~16 mantissa bits.
+// MODE 3 (FP32): act and W in FP32 on v_mfma_f32_16x16x4_f32 (the FP32
+//   matrix rate equals the FP32 vector rate on gfx950, and the matrix cores
+//   run beside the VALU's elementwise work).  The act tile is FP32 in LDS;
+//   K is permuted so that lane group q = l >> 4 supplies k = q*Gp/4 + t in
+//   the MFMA numbered t, and a lane reads four consecutive t with one
+//   16-byte LDS load.
 // AND / OR gates stay in FP32 on the VALU from an FP32 copy of their
 // sources' Hill values.
 //
@@ -101,11 +107,17 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
     const int rg = l >> 4;                 // row group: cells 4*rg .. 4*rg+3 of the tile
     const int par = l & 1;
 
-    // LDS: act tiles [plane][buffer][16 rows][Gp + 8] bf16, then FP32 gate Hill values.
-    const int rowstride = Gp + 8;
+    // LDS: act tiles [plane][buffer][16 rows][Gp + 8] bf16 (FP32: [buffer][16 rows][Gp + 4] f32, rows
+    // 16 bytes apart in bank order so that the 16-byte reads of 16 lanes hit distinct banks), then FP32
+    // gate Hill values.
+    constexpr bool F32 = MODE == 3;
+    const int rowstride = F32 ? Gp + 4 : Gp + 8;
     const int tile_elems = 16 * rowstride;
+    const int KT = Gp / 4;                 // FP32 mode: MFMAs per gene block and step
     __bf16* act_t = (__bf16*)dlds;
-    float* ghill = (float*)(dlds + (size_t)(MODE == 2 ? 4 : 2) * tile_elems * sizeof(__bf16));
+    float* act_f = (float*)dlds;
+    float* ghill = (float*)(dlds + (F32 ? (size_t)2 * tile_elems * sizeof(float)
+                                        : (size_t)(MODE == 2 ? 4 : 2) * tile_elems * sizeof(__bf16)));
 
     const int32_t* __restrict__ gate_off = (const int32_t*)(B + B[H_OFF_GATEOFF]);
     const uint32_t* __restrict__ gates = B + B[H_OFF_GATES];
@@ -116,7 +128,8 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
     // draws), lives in registers: strength*beta, OR flag and the offsets of
     // its sources' Hill rows (AND reads act rows, OR rep rows).  Other gate
     // layouts take the loop over the blob's records (gslow).
-    bf16x8 whi[NB][kDenseMaxKC], wlo[NB][kDenseMaxKC];
+    bf16x8 whi[NB][F32 ? 1 : kDenseMaxKC], wlo[NB][F32 ? 1 : kDenseMaxKC];
+    float wf[NB][F32 ? 4 * kDenseMaxKC * 2 : 1];      // FP32: W[16 blk + (l & 15)][(l >> 4) KT + t]
     int g[NB], slot[NB], q0[NB], q1[NB];
     bool gl[NB];
     float gw[NB];
@@ -138,11 +151,18 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
         g[b] = blk * 16 + (l & 15);
         gl[b] = g[b] < G;
         const int gc = gl[b] ? g[b] : 0;
+        if constexpr (F32) {
+            const float* __restrict__ wp = (const float*)wpool + off_w * 4;
 #pragma unroll
-        for (int kc = 0; kc < kDenseMaxKC; ++kc) {
-            if (kc < KC) {
-                whi[b][kc] = wpool[off_w + ((size_t)blk * KC + kc) * 64 + l];
-                if (MODE == 2) wlo[b][kc] = wpool[off_w + ((size_t)NBLK * KC + (size_t)blk * KC + kc) * 64 + l];
+            for (int t = 0; t < 8 * kDenseMaxKC; ++t)
+                if (t < KT) wf[b][t] = wp[((size_t)blk * KT + t) * 64 + l];
+        } else {
+#pragma unroll
+            for (int kc = 0; kc < kDenseMaxKC; ++kc) {
+                if (kc < KC) {
+                    whi[b][kc] = wpool[off_w + ((size_t)blk * KC + kc) * 64 + l];
+                    if (MODE == 2) wlo[b][kc] = wpool[off_w + ((size_t)NBLK * KC + (size_t)blk * KC + kc) * 64 + l];
+                }
             }
         }
         // Padding genes (g >= G) get beta = bias = 0, zero weight rows and
@@ -206,6 +226,7 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
         const int buf = s & 1;
         __bf16* T0 = act_t + (size_t)buf * tile_elems;
         __bf16* T1 = act_t + (size_t)(2 + buf) * tile_elems;    // low plane (BF16X3)
+        float* TF = act_f + (size_t)buf * tile_elems;           // FP32 mode
         float* GH = ghill + (size_t)buf * 2 * S * 16;
 
 #pragma unroll
@@ -223,9 +244,13 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
                     xn = fminf(xn, 1e30f);
                     const float rep = __builtin_amdgcn_rcpf(1.0f + xn);
                     const float act = xn * rep;
-                    const __bf16 hi = (__bf16)act;
-                    T0[row(v) * rowstride + g[b]] = hi;
-                    if (MODE == 2) T1[row(v) * rowstride + g[b]] = (__bf16)(act - (float)hi);
+                    if constexpr (F32) {
+                        TF[row(v) * rowstride + g[b]] = act;
+                    } else {
+                        const __bf16 hi = (__bf16)act;
+                        T0[row(v) * rowstride + g[b]] = hi;
+                        if (MODE == 2) T1[row(v) * rowstride + g[b]] = (__bf16)(act - (float)hi);
+                    }
                     if (slot[b] >= 0) {
                         GH[(2 * slot[b]) * 16 + row(v)] = act;
                         GH[(2 * slot[b] + 1) * 16 + row(v)] = rep;
@@ -243,11 +268,33 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
         f32x4 acc[NB];
 #pragma unroll
         for (int b = 0; b < NB; ++b) acc[b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        if constexpr (F32) {
+            // Two accumulation chains per block, summed after the loop.
+            f32x4 acc2[NB];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) acc2[b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            const float* arow = TF + (l & 15) * rowstride + rg * KT;
+#pragma unroll
+            for (int t = 0; t < 8 * kDenseMaxKC; t += 4) {
+                if (t < KT) {
+                    const f32x4 a4 = *(const f32x4*)(arow + t);
+#pragma unroll
+                    for (int b = 0; b < NB; ++b) {
+                        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[0], wf[b][t], acc[b], 0, 0, 0);
+                        acc2[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[1], wf[b][t + 1], acc2[b], 0, 0, 0);
+                        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[2], wf[b][t + 2], acc[b], 0, 0, 0);
+                        acc2[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[3], wf[b][t + 3], acc2[b], 0, 0, 0);
+                    }
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < NB; ++b) acc[b] += acc2[b];
+        }
         const __bf16* arow0 = T0 + (l & 15) * rowstride + 8 * rg;
         const __bf16* arow1 = T1 + (l & 15) * rowstride + 8 * rg;
 #pragma unroll
         for (int kc = 0; kc < kDenseMaxKC; ++kc) {
-            if (kc < KC) {
+            if (!F32 && kc < KC) {
                 const bf16x8 a = *(const bf16x8*)(arow0 + 32 * kc);
                 bf16x8 alo;
                 if (MODE == 2) alo = *(const bf16x8*)(arow1 + 32 * kc);
@@ -404,8 +451,9 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
 int dense_nb(int mode) { return mode == 2 ? 2 : 1; }
 
 size_t dense_lds_bytes(int Gp, int S, int mode) {
-    const size_t tile = (size_t)16 * (Gp + 8) * 2;
-    return (size_t)(mode == 2 ? 4 : 2) * tile + (size_t)2 * 2 * S * 16 * sizeof(float);
+    const size_t tiles = mode == 3 ? (size_t)2 * 16 * (Gp + 4) * sizeof(float)
+                                   : (size_t)(mode == 2 ? 4 : 2) * 16 * (Gp + 8) * 2;
+    return tiles + (size_t)2 * 2 * S * 16 * sizeof(float);
 }
 
 template <int MODE, int NB>
@@ -434,7 +482,8 @@ static hipError_t dense_attr(size_t bytes) {
 hipError_t set_dense_lds_limit(size_t bytes) {
     hipError_t e1 = dense_attr<1, 1>(bytes);
     hipError_t e2 = dense_attr<2, 2>(bytes);
-    return e1 != hipSuccess ? e1 : e2;
+    hipError_t e3 = dense_attr<3, 1>(bytes);
+    return e1 != hipSuccess ? e1 : e2 != hipSuccess ? e2 : e3;
 }
 
 hipError_t launch_dense(int mode, bool tele, bool inject, bool clamp, int NW, int n_tiles, const uint32_t* blobs,
@@ -444,6 +493,7 @@ hipError_t launch_dense(int mode, bool tele, bool inject, bool clamp, int NW, in
     if (n_tiles <= 0) return hipSuccess;
     const void* f = mode == 1 ? dense_fn<1, 1>(tele, inject, clamp)
                   : mode == 2 ? dense_fn<2, 2>(tele, inject, clamp)
+                  : mode == 3 ? dense_fn<3, 1>(tele, inject, clamp)
                               : nullptr;
     if (!f) return hipErrorInvalidValue;
     const bf16x8* w = (const bf16x8*)wpool;

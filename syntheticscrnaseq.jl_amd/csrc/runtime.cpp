@@ -172,7 +172,7 @@ int validate_inputs(const scrna_network* net, const scrna_kinetics* kin, const s
     if (run->readout < 0 || run->readout > 2) return set_error(SCRNA_ERR_INVALID, "readout must be 0, 1 or 2");
     if (run->init_mode < 0 || run->init_mode > 1) return set_error(SCRNA_ERR_INVALID, "init_mode must be 0 or 1");
     if (run->snapshot_every < 0) return set_error(SCRNA_ERR_INVALID, "snapshot_every must be >= 0");
-    if (run->precision < 0 || run->precision > 3) return set_error(SCRNA_ERR_INVALID, "unknown precision %d", run->precision);
+    if (run->precision < 0 || run->precision > 4) return set_error(SCRNA_ERR_INVALID, "unknown precision %d", run->precision);
     if (run->algorithm != SCRNA_ALG_CLE && run->algorithm != SCRNA_ALG_BINOMIAL_TAU_LEAP)
         return set_error(SCRNA_ERR_INVALID, "unknown algorithm %d", run->algorithm);
     if (run->reserved0 != 0) return set_error(SCRNA_ERR_INVALID, "scrna_run.reserved0 must be 0");
@@ -255,10 +255,12 @@ int compile_network(const scrna_network* net, const scrna_kinetics* kin, const s
     }
     out->CH = CH;
     out->NW = NW;
-    // Dense MFMA path: the additive regulation becomes act * W^T + b.
-    const bool dense = run->precision == SCRNA_PREC_BF16 || run->precision == SCRNA_PREC_BF16X3;
+    // Dense MFMA path: the additive regulation becomes act * W^T + b.  Kernel
+    // modes: 1 BF16, 2 BF16X3, 3 FP32.
+    const bool dense = run->precision == SCRNA_PREC_BF16 || run->precision == SCRNA_PREC_BF16X3 ||
+                       run->precision == SCRNA_PREC_FP32_MFMA;
     out->dense = dense;
-    out->mode = run->precision;
+    out->mode = run->precision == SCRNA_PREC_FP32_MFMA ? 3 : run->precision;
     out->Gp = (G + 31) / 32 * 32;
     std::vector<double> Wd, bias;
     if (dense) {
@@ -547,7 +549,19 @@ int compile_network(const scrna_network* net, const scrna_kinetics* kin, const s
     b[H_OFF_BIAS] = (uint32_t)b.size();
     for (int g = 0; g < G; ++g) b.push_back(f2w(dense ? (float)bias[g] : 0.0f));
     align16();
-    if (dense) {
+    if (dense && out->mode == 3) {
+        // FP32 W^T as v_mfma_f32_16x16x4_f32 B operands: [gene block][t][lane] f32, lane l
+        // of block blk holding W[16 blk + (l&15)][(l>>4) Gp/4 + t] (the kernel's K order).
+        const int Gp = out->Gp, NBLK = Gp / 16, KT = Gp / 4;
+        out->wpack.assign((size_t)NBLK * KT * 64 * 2, 0);
+        for (int blk = 0; blk < NBLK; ++blk)
+            for (int t = 0; t < KT; ++t)
+                for (int l = 0; l < 64; ++l) {
+                    const int gi = 16 * blk + (l & 15), gj = (l >> 4) * KT + t;
+                    const float wv = (gi < G && gj < G) ? (float)Wd[(size_t)gi * G + gj] : 0.0f;
+                    std::memcpy(&out->wpack[(((size_t)blk * KT + t) * 64 + l) * 2], &wv, 4);
+                }
+    } else if (dense) {
         // W^T as MFMA B operands: [plane][gene block][K chunk][lane][8] bf16,
         // lane l of block blk / chunk kc holding W[16 blk + (l&15)][32 kc + 8 (l>>4) + e].
         const int Gp = out->Gp, NBLK = Gp / 16, KC = Gp / 32;

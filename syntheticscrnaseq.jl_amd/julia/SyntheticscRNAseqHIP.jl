@@ -64,7 +64,8 @@ struct COutput
 end
 
 const READOUT = Dict(:mrna => Int32(0), :protein => Int32(1), :both => Int32(2))
-const PRECISION = Dict(:fp32 => Int32(0), :bf16 => Int32(1), :bf16x3 => Int32(2))
+const PRECISION = Dict(:fp32 => Int32(0), :bf16 => Int32(1), :bf16x3 => Int32(2), :fp32_unfused => Int32(3),
+                       :fp32_mfma => Int32(4))
 
 # ---- context ------------------------------------------------------------------
 const CTX = Ref{Ptr{Cvoid}}(C_NULL)

@@ -1,7 +1,8 @@
 """Dense (MFMA) regulation path against the FP64 oracle.
 
-BF16X3 recovers ~16 mantissa bits of the contraction, BF16 keeps 8; both
-are compared with the oracle on identical noise.  The G=256 check states the
+BF16X3 recovers ~16 mantissa bits of the contraction, BF16 keeps 8, FP32
+on the matrix cores keeps FP32 products and sums; all are compared with the
+oracle on identical noise.  The G=256 check states the
 per-gene mean and variance error over a cell subset that the oracle replays
 with the identical Philox stream (the workload's acceptance band is 1%).
 """
@@ -20,7 +21,7 @@ def relerr(a, b):
     return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1.0)))
 
 
-@pytest.mark.parametrize("precision,tol", [("bf16x3", 2e-4), ("bf16", 2e-2)])
+@pytest.mark.parametrize("precision,tol", [("bf16x3", 2e-4), ("bf16", 2e-2), ("fp32_mfma", 1e-5)])
 @pytest.mark.parametrize("G,cells", [(40, 50), (96, 33)])
 def test_dense_injected_parity(ctx, precision, tol, G, cells):
     rng = np.random.default_rng(G)
@@ -54,7 +55,7 @@ def test_dense_telegraph_matches_sparse_fp32(ctx):
 
 
 @pytest.mark.parametrize("precision,mean_tol,var_tol", [("bf16x3", 1e-3, 2e-3), ("bf16", 1e-2, 1e-2),
-                                                        ("fp32", 1e-3, 2e-3)])
+                                                        ("fp32", 1e-3, 2e-3), ("fp32_mfma", 1e-3, 2e-3)])
 def test_g256_precision_moments(ctx, precision, mean_tol, var_tol):
     net = sc.samplers.dense_g256_network()
     cells, dt, steps, seed = 384, 0.005, 300, 17
