@@ -290,22 +290,44 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
 #pragma unroll
             for (int b = 0; b < NB; ++b) acc[b] += acc2[b];
         }
-        const __bf16* arow0 = T0 + (l & 15) * rowstride + 8 * rg;
-        const __bf16* arow1 = T1 + (l & 15) * rowstride + 8 * rg;
+        if constexpr (!F32) {
+            // With the chunk count a compile-time constant, the A reads of all
+            // chunks issue ahead of the MFMAs, and even and odd chunks
+            // accumulate in two independent chains.
+            const __bf16* arow0 = T0 + (l & 15) * rowstride + 8 * rg;
+            const __bf16* arow1 = T1 + (l & 15) * rowstride + 8 * rg;
+            auto contract = [&](auto kcn) {
+                constexpr int KCN = decltype(kcn)::value;
+                f32x4 acc2[NB];
 #pragma unroll
-        for (int kc = 0; kc < kDenseMaxKC; ++kc) {
-            if (!F32 && kc < KC) {
-                const bf16x8 a = *(const bf16x8*)(arow0 + 32 * kc);
-                bf16x8 alo;
-                if (MODE == 2) alo = *(const bf16x8*)(arow1 + 32 * kc);
+                for (int b = 0; b < NB; ++b) acc2[b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-                for (int b = 0; b < NB; ++b) {
-                    acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, whi[b][kc], acc[b], 0, 0, 0);
-                    if (MODE == 2) {
-                        acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, whi[b][kc], acc[b], 0, 0, 0);
-                        acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wlo[b][kc], acc[b], 0, 0, 0);
+                for (int kc = 0; kc < KCN; ++kc) {
+                    const bf16x8 a = *(const bf16x8*)(arow0 + 32 * kc);
+                    bf16x8 alo;
+                    if (MODE == 2) alo = *(const bf16x8*)(arow1 + 32 * kc);
+#pragma unroll
+                    for (int b = 0; b < NB; ++b) {
+                        f32x4& c = (kc & 1) ? acc2[b] : acc[b];
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, whi[b][kc], c, 0, 0, 0);
+                        if (MODE == 2) {
+                            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, whi[b][kc], c, 0, 0, 0);
+                            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wlo[b][kc], c, 0, 0, 0);
+                        }
                     }
                 }
+#pragma unroll
+                for (int b = 0; b < NB; ++b) acc[b] += acc2[b];
+            };
+            switch (KC) {
+                case 8: contract(std::integral_constant<int, 8>{}); break;
+                case 7: contract(std::integral_constant<int, 7>{}); break;
+                case 6: contract(std::integral_constant<int, 6>{}); break;
+                case 5: contract(std::integral_constant<int, 5>{}); break;
+                case 4: contract(std::integral_constant<int, 4>{}); break;
+                case 3: contract(std::integral_constant<int, 3>{}); break;
+                case 2: contract(std::integral_constant<int, 2>{}); break;
+                default: contract(std::integral_constant<int, 1>{}); break;
             }
         }
 
@@ -355,7 +377,9 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
                 }
                 on ^= (uint32_t)flip << bit;
             }
-            const bool o = (on >> (4 * b + v)) & 1u;
+            // Without telegraph genes every live gene is ON, and padding genes have
+            // zero drive (reg = 0), so the gate on the promoter bit is dropped.
+            const bool o = !tele || ((on >> (4 * b + v)) & 1u);
             const float tx = o ? reg : 0.0f;
             const float mm = m[b][v];
             const float var_m = fmaxf(fmaf(mu_m, CLAMP ? fmaxf(mm, 0.0f) : mm, tx), 0.0f);
