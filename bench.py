@@ -189,7 +189,7 @@ def cpu_baselines(target_s: float = 4.0) -> tuple:
     kin = samplers.WORKLOAD_KINETICS
     cases = [("G=5 workload (5000 cells, 5000 steps, zero init)", w0["net"], w0["kin"], w0["dt"], 5000, 5000, "zeros"),
              ("G=20 telegraph workload slice (5000 steps)", w1["net"], w1["kin"], w1["dt"], 5000, 100_000, "steady"),
-             ("G=256 precision workload slice (400 of 2000 steps, dt=0.005)", g256, kin, 0.005, 400, 65536, "steady")]
+             ("G=256 precision workload slice (dt=0.005)", g256, kin, 0.005, 400, 65536, "steady")]
     entries = []
     primary = None
     for case in cases:
@@ -324,7 +324,7 @@ def run_g256(a, sc) -> dict:
     out = {"workload": "G=256 additive network, 25% dense A, 8 AND + 8 OR, 65536 cells, 2000 steps dt=0.005",
            "cells": cells, "steps": steps, "results": {}}
     subsets = {}
-    precs = [p for p in ("fp32_unfused", "fp32", "fp32_mfma", "bf16", "bf16x3") if p in a.precisions.split(",")]
+    precs = [p for p in ("fp32_unfused", "fp32_valu", "fp32_mfma", "bf16", "bf16x3") if p in a.precisions.split(",")]
     for prec in precs:
         plan = sc.Plan([net], sc.CLE(dt), kin, cell_num=cells, T=T, readout="both", seed=777, precision=prec)
         # the timed forms of `value` run the requested count; comparison forms at most 5 (they take up to 1.3 s each)
@@ -366,13 +366,13 @@ def run_g256(a, sc) -> dict:
             out["results"][prec]["vs_fp64_replay"] = errs
         out["replay_cells"] = n_sub
     res = out["results"]
-    fused = [p for p in ("fp32_mfma", "fp32") if p in res]
+    fused = [p for p in ("fp32_mfma", "fp32_valu") if p in res]
     best = max(fused, key=lambda p: res[p]["cgs_per_s"]) if fused else None
     out["fp32_fused_form"] = {"fp32_mfma": "FP32 dense contraction on the matrix cores (v_mfma_f32_16x16x4_f32), "
                                            "FP32 products and accumulation, fused stepper",
-                              "fp32": "sparse FP32 VALU stepper, 16 genes per lane, in-edges streamed per step",
+                              "fp32_valu": "sparse FP32 VALU stepper, 16 genes per lane, in-edges streamed per step",
                               None: None}[best]
-    for prec in ("fp32", "fp32_mfma", "bf16", "bf16x3"):
+    for prec in ("fp32_valu", "fp32_mfma", "bf16", "bf16x3"):
         if prec not in res:
             continue
         if best:
@@ -623,7 +623,7 @@ def main() -> int:
     ap.add_argument("--tile", type=int, default=0, help="genes per lane for the FP32 stepper (2, 4, 8, 16)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--subset", type=int, default=256, help="g256: cells replayed by the FP64 oracle")
-    ap.add_argument("--precisions", default="fp32_unfused,fp32,fp32_mfma,bf16,bf16x3",
+    ap.add_argument("--precisions", default="fp32_unfused,fp32_valu,fp32_mfma,bf16,bf16x3",
                     help="g256: comma-separated forms")
     ap.add_argument("--networks", type=int, default=4096, help="batch: number of networks")
     ap.add_argument("--T", type=float, default=None, help="batch: horizon")

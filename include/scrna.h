@@ -49,13 +49,17 @@ typedef enum scrna_readout {
 } scrna_readout;
 
 typedef enum scrna_precision {
-    SCRNA_PREC_FP32 = 0,        /* FP32 state, regulation on the VALU */
+    SCRNA_PREC_FP32 = 0,        /* FP32 state and regulation; the library picks the form: the dense matrix-core
+                                   contraction (SCRNA_PREC_FP32_MFMA) for a CLE network without population
+                                   dynamics with G >= 128 and at least 16 additive in-edges per gene on
+                                   average, the sparse VALU stepper (SCRNA_PREC_FP32_VALU) otherwise */
     SCRNA_PREC_BF16 = 1,        /* dense regulation on BF16 MFMA, FP32 accumulate */
     SCRNA_PREC_BF16X3 = 2,      /* dense regulation as hi*hi + hi*lo + lo*hi BF16 MFMA */
     SCRNA_PREC_FP32_UNFUSED = 3,/* the reference GPU structure (ext:87-108): one kernel per
                                    broadcast, two SGEMMs per step, state in HBM -- a baseline */
-    SCRNA_PREC_FP32_MFMA = 4    /* dense regulation in FP32 on the matrix cores (FP32 products and
+    SCRNA_PREC_FP32_MFMA = 4,   /* dense regulation in FP32 on the matrix cores (FP32 products and
                                    accumulation), FP32 state */
+    SCRNA_PREC_FP32_VALU = 5    /* sparse regulation in FP32 on the VALU (in-edges per gene), FP32 state */
 } scrna_precision;
 
 /* Simulation algorithm of a run (the reference dispatches on the algorithm

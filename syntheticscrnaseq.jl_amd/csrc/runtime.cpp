@@ -172,12 +172,12 @@ int validate_inputs(const scrna_network* net, const scrna_kinetics* kin, const s
     if (run->readout < 0 || run->readout > 2) return set_error(SCRNA_ERR_INVALID, "readout must be 0, 1 or 2");
     if (run->init_mode < 0 || run->init_mode > 1) return set_error(SCRNA_ERR_INVALID, "init_mode must be 0 or 1");
     if (run->snapshot_every < 0) return set_error(SCRNA_ERR_INVALID, "snapshot_every must be >= 0");
-    if (run->precision < 0 || run->precision > 4) return set_error(SCRNA_ERR_INVALID, "unknown precision %d", run->precision);
+    if (run->precision < 0 || run->precision > 5) return set_error(SCRNA_ERR_INVALID, "unknown precision %d", run->precision);
     if (run->algorithm != SCRNA_ALG_CLE && run->algorithm != SCRNA_ALG_BINOMIAL_TAU_LEAP)
         return set_error(SCRNA_ERR_INVALID, "unknown algorithm %d", run->algorithm);
     if (run->reserved0 != 0) return set_error(SCRNA_ERR_INVALID, "scrna_run.reserved0 must be 0");
     if (run->algorithm == SCRNA_ALG_BINOMIAL_TAU_LEAP) {
-        if (run->precision != SCRNA_PREC_FP32)
+        if (run->precision != SCRNA_PREC_FP32 && run->precision != SCRNA_PREC_FP32_VALU)
             return set_error(SCRNA_ERR_UNSUPPORTED, "the tau-leap computes propensities in FP64; precision must be 0");
         if (run->injected_noise)
             return set_error(SCRNA_ERR_INVALID, "injected noise applies to the CLE only");
@@ -617,6 +617,23 @@ void plan_release(scrna_plan* p) {
     p->graph = nullptr;
 }
 
+// Precision policy: SCRNA_PREC_FP32 runs the dense FP32 contraction on the
+// matrix cores once the network is large and dense enough that streaming
+// in-edges on the VALU costs more.  Measured on the G=256 network with 64
+// additive in-edges per gene: sparse VALU stepper 670 ms, matrix-core kernel
+// 185 ms; sparse networks (a few in-edges per gene) stay on the VALU, whose
+// cost grows with the in-degree rather than with G.
+static int effective_precision(const scrna_network& net, const scrna_run& run, bool population) {
+    if (run.precision == SCRNA_PREC_FP32_VALU) return SCRNA_PREC_FP32;
+    if (run.precision != SCRNA_PREC_FP32 || run.algorithm != SCRNA_ALG_CLE || population || net.G < 128)
+        return run.precision;
+    int64_t nnz = 0;
+    for (int i = 0; i < net.G; ++i)
+        for (int j = 0; j < net.G; ++j)
+            if (i != j && net.A[i + (size_t)j * net.G] * net.basals[i] != 0.0) ++nnz;
+    return nnz >= 16 * (int64_t)net.G ? SCRNA_PREC_FP32_MFMA : SCRNA_PREC_FP32;
+}
+
 int plan_alloc(scrna_plan* p, void** ptr, size_t bytes) {
     if (p->arena) {
         Arena& a = *p->arena;
@@ -664,10 +681,12 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
     for (int i = 0; i < n; ++i) {
         CompiledNet& cn = P->nets[i];
         const int ch = force_ch ? force_ch : ctx_tile_for(ctx, nets[i].G);   // autotuner table
-        int rc = compile_network(&nets[i], &kins[i], &runs[i], pops ? &pops[i] : nullptr, ch, &cn);
+        scrna_run run = runs[i];
+        run.precision = effective_precision(nets[i], run, pops && pops[i].enabled);
+        int rc = compile_network(&nets[i], &kins[i], &run, pops ? &pops[i] : nullptr, ch, &cn);
         if (rc) return rc;
         cn.readout = runs[i].readout;
-        cn.precision = runs[i].precision;
+        cn.precision = run.precision;
         cn.capture = caps && caps[i].enabled;
         if (cn.capture) cn.cap = caps[i];
         cn.seed = runs[i].seed;

@@ -574,12 +574,15 @@ int scrna_submit(scrna_server* S, const scrna_request* req, int64_t* ticket) {
     const scrna_capture* cap = req->capture.enabled ? &req->capture : nullptr;
     int rc = validate_inputs(&req->net, &req->kin, &req->run, nullptr, cap);
     if (rc) return rc;
-    if (req->run.precision != SCRNA_PREC_FP32 || req->run.snapshot_every != 0 || req->run.injected_noise ||
+    if ((req->run.precision != SCRNA_PREC_FP32 && req->run.precision != SCRNA_PREC_FP32_VALU) ||
+        req->run.snapshot_every != 0 || req->run.injected_noise ||
         req->run.algorithm != SCRNA_ALG_CLE)
         return set_error(SCRNA_ERR_UNSUPPORTED,
                          "served requests run the FP32 CLE without snapshots or injected noise");
     std::unique_ptr<Req> r(new Req());
-    rc = compile_network(&req->net, &req->kin, &req->run, nullptr, 0, &r->cn);
+    scrna_run run = req->run;
+    run.precision = SCRNA_PREC_FP32;     // served requests run the sparse FP32 stepper
+    rc = compile_network(&req->net, &req->kin, &run, nullptr, 0, &r->cn);
     if (rc) return rc;
     r->cn.seed = req->run.seed;
     r->cn.net_id = req->run.net_id;

@@ -65,7 +65,7 @@ end
 
 const READOUT = Dict(:mrna => Int32(0), :protein => Int32(1), :both => Int32(2))
 const PRECISION = Dict(:fp32 => Int32(0), :bf16 => Int32(1), :bf16x3 => Int32(2), :fp32_unfused => Int32(3),
-                       :fp32_mfma => Int32(4))
+                       :fp32_mfma => Int32(4), :fp32_valu => Int32(5))
 
 # ---- context ------------------------------------------------------------------
 const CTX = Ref{Ptr{Cvoid}}(C_NULL)

@@ -23,7 +23,7 @@ SCRNA_ERR_TIMEOUT = -5
 SCRNA_ERR_UNSUPPORTED = -6
 
 READOUT = {"mrna": 0, "protein": 1, "both": 2}
-PRECISION = {"fp32": 0, "bf16": 1, "bf16x3": 2, "fp32_unfused": 3, "fp32_mfma": 4}
+PRECISION = {"fp32": 0, "bf16": 1, "bf16x3": 2, "fp32_unfused": 3, "fp32_mfma": 4, "fp32_valu": 5}
 
 c_double_p = C.POINTER(C.c_double)
 c_int32_p = C.POINTER(C.c_int32)

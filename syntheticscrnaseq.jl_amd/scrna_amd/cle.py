@@ -72,7 +72,7 @@ def _run(alg, cell_num: int, T: float, readout: str, init: str, seed, net_id: in
         raise ValueError("init must be 'steady' (the steady state, cle.jl:52-53 / tauleap_binomial.jl:61-62) "
                          "or 'zeros'")
     if precision not in _lib.PRECISION:
-        raise ValueError("precision must be 'fp32', 'fp32_mfma', 'bf16', 'bf16x3' or 'fp32_unfused'")
+        raise ValueError("precision must be 'fp32', 'fp32_valu', 'fp32_mfma', 'bf16', 'bf16x3' or 'fp32_unfused'")
     keep = []
     inj_p = None
     if injected is not None:

@@ -55,7 +55,7 @@ def test_dense_telegraph_matches_sparse_fp32(ctx):
 
 
 @pytest.mark.parametrize("precision,mean_tol,var_tol", [("bf16x3", 1e-3, 2e-3), ("bf16", 1e-2, 1e-2),
-                                                        ("fp32", 1e-3, 2e-3), ("fp32_mfma", 1e-3, 2e-3)])
+                                                        ("fp32_valu", 1e-3, 2e-3), ("fp32_mfma", 1e-3, 2e-3)])
 def test_g256_precision_moments(ctx, precision, mean_tol, var_tol):
     net = sc.samplers.dense_g256_network()
     cells, dt, steps, seed = 384, 0.005, 300, 17
@@ -85,3 +85,18 @@ def test_reference_structured_fp32_matches_oracle(ctx):
     err = relerr(y, np.concatenate([ref["m"].T, ref["p"].T], axis=1))
     print(f"unfused max rel err {err:.3e}")
     assert err <= 2e-4
+
+
+def test_fp32_precision_policy(ctx):
+    """Default FP32 picks the matrix-core contraction for the dense G=256 network
+    (64 additive in-edges per gene) and the sparse VALU stepper for a sparse
+    G=200 network: each is bit-identical to the form it picks."""
+    dense = sc.samplers.dense_g256_network()
+    kw = dict(cell_num=96, T=0.2, readout="both", seed=5)
+    a = sc.simulate(dense, sc.CLE(0.005), KIN, precision="fp32", **kw)
+    np.testing.assert_array_equal(a, sc.simulate(dense, sc.CLE(0.005), KIN, precision="fp32_mfma", **kw))
+    assert not np.array_equal(a, sc.simulate(dense, sc.CLE(0.005), KIN, precision="fp32_valu", **kw))
+    sparse = sc.samplers.mixed_network(200, np.random.default_rng(6), n_tf=(4, 5), targets_per_tf=(3, 6),
+                                       n_and=(1, 2), n_or=(1, 2))
+    b = sc.simulate(sparse, sc.CLE(0.01), KIN, precision="fp32", **kw)
+    np.testing.assert_array_equal(b, sc.simulate(sparse, sc.CLE(0.01), KIN, precision="fp32_valu", **kw))
