@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SCRNA_ABI_VERSION 2
+#define SCRNA_ABI_VERSION 3
 
 typedef enum scrna_status {
     SCRNA_OK = 0,
@@ -108,6 +108,9 @@ typedef struct scrna_population {
     int32_t div_check_interval;
     int32_t partition;            /* 0: halve (cle.jl:249-261, ext:494-505); 1: Binomial(round(x), 1/2) (population.jl:166-192) */
     int32_t replace_mode;         /* 0: uniform over 1:N (population.jl:196, ext:509); 1: exclude mother (cle.jl:265-268) */
+    double growth_noise;          /* sigma of population.jl:128-139: with the binomial tau-leap each cell's volume
+                                     grows by exp((growth_rate + sigma*xi)*dt) per step (tauleap_binomial.jl:253);
+                                     the population CLE grows deterministically and ignores it (cle.jl:169) */
 } scrna_population;
 
 /* CaptureModel (population.jl:57-74). */

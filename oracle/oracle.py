@@ -37,7 +37,7 @@ class OrcKinetics(C.Structure):
 class OrcPopulation(C.Structure):
     _fields_ = [("enabled", C.c_int32), ("growth_rate", C.c_double), ("V_div", C.c_double),
                 ("V_init_lo", C.c_double), ("V_init_hi", C.c_double), ("div_check_interval", C.c_int32),
-                ("partition", C.c_int32), ("replace_mode", C.c_int32)]
+                ("partition", C.c_int32), ("replace_mode", C.c_int32), ("growth_noise", C.c_double)]
 
 
 class OrcRun(C.Structure):
@@ -224,7 +224,7 @@ def _orc_pop(pop):
     part = {"halve": 0, "binomial": 1}[pop.partition]
     rep = {"uniform": 0, "exclude_mother": 1}[pop.replace]
     return OrcPopulation(1, pop.growth_rate, pop.V_div, pop.V_init[0], pop.V_init[1], pop.div_check_interval,
-                         part, rep)
+                         part, rep, getattr(pop, "growth_noise", 0.0))
 
 
 def divide(pop, m_gc, p_gc, V, *, seed, step, net_id=0):

@@ -53,9 +53,10 @@ enum {
     ORC_P_PARTITION = 5,   /* ctr = (mother, step, block), per (species,gene) sub-stream in word 3 bits */
     ORC_P_REPLACE = 6,     /* ctr = (mother, step, 0) */
     ORC_P_VINIT = 7,       /* ctr = (cell, 0, 0) */
-    ORC_P_TAULEAP = 8      /* ctr = (cell, step, (gene << 16) + block): one 52-bit uniform stream per
+    ORC_P_TAULEAP = 8,     /* ctr = (cell, step, (gene << 16) + block): one 52-bit uniform stream per
                               (cell, step, gene), read by transcription, mRNA decay, translation and
                               protein decay in that order */
+    ORC_P_GROWTH = 9       /* ctr = (cell, step, 0): the volume growth noise of one cell and step */
 };
 
 static inline void orc_draw(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
@@ -1086,7 +1087,16 @@ int orc_tauleap_population(const orc_network* net, const orc_kinetics* kin, cons
                 Pc[g] = (double)p;
             }
         }
-        for (int64_t c = 0; c < N; ++c) V[c] *= growth;                         /* :253; population.jl:136-137 */
+        for (int64_t c = 0; c < N; ++c) {                                       /* :253 */
+            if (pop->growth_noise > 0.0) {                                      /* population.jl:130-134 */
+                uint32_t u[4];
+                orc_draw(run->seed, (uint32_t)c, (uint32_t)s, 0, c3base | ORC_P_GROWTH, u);
+                const double xi = sqrt(-2.0 * log(orc_u52(u[0], u[1]))) * cos(6.283185307179586 * orc_u52(u[2], u[3]));
+                V[c] *= exp((pop->growth_rate + pop->growth_noise * xi) * dt);
+            } else {
+                V[c] *= growth;                                                 /* population.jl:136-137 */
+            }
+        }
         if (((s + 1) % pop->div_check_interval) == 0)                            /* :256-258 */
             orc_division_check(pop, G, N, run->seed, c3base, s, M, P, V, on, div_idx, dM, dP, dV, dOn, 0.0, NULL,
                                NULL);

@@ -73,6 +73,7 @@ typedef struct orc_population {
     int32_t div_check_interval;
     int32_t partition;           /* 0: halve continuously (cle.jl:249-261); 1: Binomial(round(x), 1/2) */
     int32_t replace_mode;        /* 0: uniform over all N (population.jl:194-202); 1: exclude mother (cle.jl:264-268) */
+    double growth_noise;         /* population.jl:128-139 (binomial tau-leap; the CLE ignores it, cle.jl:169) */
 } orc_population;
 
 typedef struct orc_run {

@@ -81,6 +81,8 @@ enum TlWord : int {
     TL_PDP,           // 1 - exp(-(mu_p + dilution) dt), tauleap_binomial.jl:157
     TL_LQP,           // log(1 - TL_PDP)
     TL_LPP,           // log(TL_PDP)
+    TL_GROWTH_RATE,   // population growth rate lambda (volume noise: V *= exp((lambda + sigma xi) dt))
+    TL_GROWTH_NOISE,  // its sigma (population.jl:128-139); 0 = deterministic growth by H_GROWTH
     TL_HEAD = 16
 };
 

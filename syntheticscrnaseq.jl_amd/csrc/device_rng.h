@@ -24,7 +24,8 @@ enum Purpose : uint32_t {
     kPartition = 5,    // (mother, step, ((2*gene+species)<<16) + block)
     kReplace = 6,      // (mother, step, 0)
     kVInit = 7,        // (cell, 0, 0)
-    kTauLeap = 8       // (cell, step, (gene << 16) + block): the tau-leap draws of one gene
+    kTauLeap = 8,      // (cell, step, (gene << 16) + block): the tau-leap draws of one gene
+    kGrowth = 9        // (cell, step, 0): the volume growth noise of one cell and step
 };
 
 struct U4 {

@@ -201,6 +201,8 @@ int validate_inputs(const scrna_network* net, const scrna_kinetics* kin, const s
             return set_error(SCRNA_ERR_INVALID, "invalid population configuration");
         if (pop->partition < 0 || pop->partition > 1 || pop->replace_mode < 0 || pop->replace_mode > 1)
             return set_error(SCRNA_ERR_INVALID, "invalid population partition / replace mode");
+        if (!(pop->growth_noise >= 0.0) || !std::isfinite(pop->growth_noise))
+            return set_error(SCRNA_ERR_INVALID, "growth_noise must be finite and >= 0");
     }
     if (cap && cap->enabled) {
         // CaptureModel asserts readout in (:mrna, :protein) (population.jl:63)
@@ -517,6 +519,8 @@ int compile_network(const scrna_network* net, const scrna_kinetics* kin, const s
         d[TL_PDP] = pdp;
         d[TL_LQP] = std::log(1.0 - pdp);
         d[TL_LPP] = std::log(pdp);
+        d[TL_GROWTH_RATE] = population ? pop->growth_rate : 0.0;
+        d[TL_GROWTH_NOISE] = population ? pop->growth_noise : 0.0;
         for (int g = 0; g < G; ++g) d.push_back(net->basals[g]);
         for (int g = 0; g < G; ++g)
             for (auto& ed : in_edges[g]) d.push_back(std::fabs(ed.second));

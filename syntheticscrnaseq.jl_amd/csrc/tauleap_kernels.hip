@@ -105,6 +105,7 @@ tauleap_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restric
     const bool pop = (flags & F_POP) != 0;
     const double growth = __longlong_as_double((long long)tl_u64(B, H_GROWTH_LO));
     const double* __restrict__ D = (const double*)(B + B[H_OFF_TL]);
+    const double g_rate = D[TL_GROWTH_RATE], g_noise = D[TL_GROWTH_NOISE];
     const double Kn = D[TL_KN], hill_n = D[TL_HILLN], k_t = D[TL_KT], dt = D[TL_DT];
     const double pdm = D[TL_PDM], pdp = D[TL_PDP];
     const double lqm[2] = {D[TL_LQM], D[TL_LPM]};
@@ -247,7 +248,17 @@ tauleap_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restric
             M(k) = mk;
             P(k) = pk;
         }
-        if (pop) V *= growth;                                                    // :253, population.jl:136-137
+        if (pop) {                                                               // :253
+            if (g_noise > 0.0) {
+                // population.jl:130-134: V *= exp((lambda + sigma xi) dt), xi ~ N(0,1) per cell and
+                // step (FP64 Box-Muller on two 52-bit uniforms of the cell's growth block)
+                const U4 u = philox4x32_10((uint32_t)cell, (uint32_t)s, 0u, (B[H_NETID] << 4) | kGrowth, k0, k1);
+                const double xi = sqrt(-2.0 * log(u52(u.x, u.y))) * cos(6.283185307179586 * u52(u.z, u.w));
+                V *= exp((g_rate + g_noise * xi) * dt);
+            } else {
+                V *= growth;                                                     // population.jl:136-137
+            }
+        }
         if (nbuf == 1) __syncthreads();
 
         if (snap_every > 0 && ((s + 1) % snap_every) == 0 && live) {

@@ -45,7 +45,7 @@ end
 
 struct CPopulation
     enabled::Int32; growth_rate::Float64; V_div::Float64; V_init_lo::Float64; V_init_hi::Float64
-    div_check_interval::Int32; partition::Int32; replace_mode::Int32
+    div_check_interval::Int32; partition::Int32; replace_mode::Int32; growth_noise::Float64
 end
 
 struct CCapture
@@ -112,10 +112,9 @@ end
 _ckin(k::KineticParams) = CKinetics(k.k_t, k.K_d, k.n, k.mu_m, k.mu_p, k.dilution)
 
 function _cpop(pop::Union{Nothing,PopulationConfig}; partition=:halve, replace=:uniform)
-    pop === nothing && return CPopulation(0, 0, 0, 0, 0, 0, 0, 0)
-    pop.growth_noise == 0.0 || error("growth_noise > 0 is not supported by the GPU CLE")
+    pop === nothing && return CPopulation(0, 0, 0, 0, 0, 0, 0, 0, 0)
     CPopulation(1, pop.growth_rate, pop.V_div, pop.V_init[1], pop.V_init[2], Int32(pop.div_check_interval),
-                partition === :binomial ? 1 : 0, replace === :exclude_mother ? 1 : 0)
+                partition === :binomial ? 1 : 0, replace === :exclude_mother ? 1 : 0, pop.growth_noise)
 end
 
 _ccap(c::Union{Nothing,CaptureModel}) = c === nothing ? CCapture(0, 0, 0, 0) :

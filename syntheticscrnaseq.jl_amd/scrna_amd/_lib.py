@@ -64,6 +64,7 @@ class ScrnaPopulation(C.Structure):
         ("div_check_interval", C.c_int32),
         ("partition", C.c_int32),
         ("replace_mode", C.c_int32),
+        ("growth_noise", C.c_double),
     ]
 
 

@@ -42,7 +42,9 @@ class PopulationConfig:
     rules: "binomial" partitioning with replacement over all N cells is
     population.jl:157-205; "halve" with "exclude_mother" is the CPU CLE
     (cle.jl:249-272); "halve" with "uniform" is the CUDA extension
-    (ext/SyntheticscRNAseqCUDAExt.jl:494-513).
+    (ext/SyntheticscRNAseqCUDAExt.jl:494-513).  ``growth_noise`` (population.jl:128-139)
+    randomises each cell's volume growth in the binomial tau-leap; the CLE grows
+    volumes deterministically and ignores it, as cle.jl:169 does.
     """
     cell_num: int = 1000
     growth_rate: float = 0.03
@@ -54,12 +56,11 @@ class PopulationConfig:
     replace: str = "uniform"
 
     def to_c(self) -> _lib.ScrnaPopulation:
-        if self.growth_noise != 0.0:
-            raise _lib.UnsupportedError("growth_noise > 0 is not supported by the GPU CLE")
         part = {"halve": 0, "binomial": 1}[self.partition]
         rep = {"uniform": 0, "exclude_mother": 1}[self.replace]
         return _lib.ScrnaPopulation(1, self.growth_rate, self.V_div, float(self.V_init[0]),
-                                    float(self.V_init[1]), int(self.div_check_interval), part, rep)
+                                    float(self.V_init[1]), int(self.div_check_interval), part, rep,
+                                    float(self.growth_noise))
 
 
 @dataclass(frozen=True)

@@ -34,7 +34,7 @@ def test_ctypes_signatures_cover_header():
 
 def test_version_calls_without_device():
     lib = _lib.load()
-    assert lib.scrna_abi_version() == 2
+    assert lib.scrna_abi_version() == 3
     assert b"gfx950" in lib.scrna_version()
 
 
@@ -44,7 +44,7 @@ STRUCTS = {
                                           "redun_strength", "k_on", "k_off"]),
     "scrna_kinetics": (_lib.ScrnaKinetics, ["k_t", "K_d", "n", "mu_m", "mu_p", "dilution"]),
     "scrna_population": (_lib.ScrnaPopulation, ["enabled", "growth_rate", "V_div", "V_init_lo", "V_init_hi",
-                                                "div_check_interval", "partition", "replace_mode"]),
+                                                "div_check_interval", "partition", "replace_mode", "growth_noise"]),
     "scrna_capture": (_lib.ScrnaCapture, ["enabled", "efficiency", "efficiency_std", "readout"]),
     "scrna_run": (_lib.ScrnaRun, ["dt", "T", "cell_num", "readout", "init_mode", "seed", "net_id", "precision",
                                   "snapshot_every", "injected_noise", "algorithm", "reserved0"]),

@@ -180,3 +180,27 @@ def test_population_reference_band(ctx):
     assert y.shape == (2000, 1)
     assert np.all(y >= 0)
     assert y.mean() > 0.5 * 100.0 and np.isfinite(y.mean())
+
+
+@pytest.mark.parametrize("partition_check", [5, 1])
+def test_population_growth_noise_matches_oracle(ctx, partition_check):
+    """growth_noise (population.jl:128-139): each cell's volume grows by
+    exp((lambda + sigma xi) dt) per step, xi from the cell's growth block.  The
+    device's FP64 log / cos / exp may differ from the host's in the last bit, so
+    volumes agree to 1e-12; the division decisions and hence every count agree
+    exactly."""
+    rng = np.random.default_rng(23)
+    net = sc.samplers.mixed_network(8, rng, n_tf=(2, 3), targets_per_tf=(2, 4), n_and=(1, 1), n_or=(1, 1),
+                                    telegraph=True)
+    pop = sc.PopulationConfig(cell_num=400, growth_rate=0.05, V_div=2.0, V_init=(0.8, 1.2),
+                              div_check_interval=partition_check, growth_noise=0.3)
+    dt, T, seed = 0.1, 30.0, 19
+    print(f"tau-leap growth noise, check every {partition_check}", flush=True)
+    res = sc.simulate(net, sc.BinomialTauLeap(dt), KIN, "gpu", pop, T=T, readout="both", seed=seed, full=True)
+    ref = orc.tauleap_population(net, KIN, pop, dt=dt, T=T, cell_num=400, seed=seed)
+    np.testing.assert_allclose(res.volumes, ref["V"], rtol=1e-12)
+    assert np.array_equal(res.y, both(ref))
+    assert not np.allclose(res.volumes, orc.tauleap_population(
+        net, KIN, sc.PopulationConfig(cell_num=400, growth_rate=0.05, V_div=2.0, V_init=(0.8, 1.2),
+                                      div_check_interval=partition_check), dt=dt, T=T, cell_num=400,
+        seed=seed)["V"])

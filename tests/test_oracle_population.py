@@ -227,3 +227,38 @@ def test_ambiguity_taint_marks_without_changing_the_run():
     assert 0 < some["taint"].sum() < every["taint"].sum()
     # with a margin that covers every value, every cell descends from some divider or divided itself
     assert every["taint"].mean() > 0.9
+
+
+def test_growth_noise_law_tauleap():
+    """population.jl:130-134: one step multiplies V by exp((lambda + sigma xi) dt):
+    log(V1/V0) has mean lambda dt and standard deviation sigma dt."""
+    net = sc.GeneNetwork(1, np.array([2.0]), np.zeros((1, 1)))
+    kin = sc.KineticParams(k_t=1.0, K_d=50.0, n=2.0, mu_m=0.1, mu_p=0.2)
+    lam, sigma, dt, N = 0.05, 0.4, 0.1, 20000
+    pop = sc.PopulationConfig(cell_num=N, growth_rate=lam, V_div=1e9, V_init=(0.8, 1.2), div_check_interval=5,
+                              growth_noise=sigma)
+    v0 = orc.tauleap_population(net, kin, pop, dt=dt, T=0.0, cell_num=N, seed=3)["V"]
+    v1 = orc.tauleap_population(net, kin, pop, dt=dt, T=dt, cell_num=N, seed=3)["V"]
+    r = np.log(v1 / v0)
+    assert abs(r.mean() - lam * dt) < 5 * sigma * dt / np.sqrt(N)
+    assert abs(r.std() / (sigma * dt) - 1) < 0.03
+    assert stats.kstest((r - lam * dt) / (sigma * dt), "norm").pvalue > 1e-4
+
+
+def test_growth_noise_ignored_by_cle_and_volume_law_kept():
+    """cle.jl:169 grows volumes deterministically whatever growth_noise is; with the
+    tau-leap the noisy growth keeps the Moran volume law within the reference's 15%
+    (test_analytical_moments.jl:394-406)."""
+    net = sc.GeneNetwork(1, np.array([2.0]), np.zeros((1, 1)))
+    kin = sc.KineticParams(k_t=1.0, K_d=50.0, n=2.0, mu_m=0.1, mu_p=0.2)
+    base = dict(cell_num=800, growth_rate=0.1, V_div=2.0, V_init=(1.0, 2.0), div_check_interval=1)
+    a = orc.cle_population(net, kin, sc.PopulationConfig(**base), dt=0.1, T=30.0, cell_num=800, seed=5)
+    b = orc.cle_population(net, kin, sc.PopulationConfig(**base, growth_noise=0.5), dt=0.1, T=30.0, cell_num=800,
+                           seed=5)
+    np.testing.assert_array_equal(a["V"], b["V"])
+    np.testing.assert_array_equal(a["m"], b["m"])
+    pop = sc.PopulationConfig(cell_num=3000, growth_rate=0.03, V_div=2.0, V_init=(1.0, 2.0), div_check_interval=1,
+                              growth_noise=0.05)
+    V = orc.tauleap_population(net, kin, pop, dt=0.1, T=1000.0, cell_num=3000, seed=42)["V"]
+    assert abs(V.mean() / (2.0 * np.log(2.0)) - 1) < 0.15
+    assert abs(np.mean(1.0 / V) / 0.75 - 1) < 0.15
