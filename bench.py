@@ -530,14 +530,16 @@ def run_batch(a, sc) -> dict:
     plan = sc.Plan(nets, sc.CLE(dt), kin, population=pops, T=T, readout="mrna", seed=99)
     t_build = time.perf_counter() - t0
     t0 = time.perf_counter()
-    plan.use_graph()
+    if not a.no_graph:
+        plan.use_graph()
     t_graph = time.perf_counter() - t0
     el, kms = timed_runs(plan, a.steps, a.warmup)
     steps = sc.n_steps(T, dt)
     cgs = float(sum(net.G * c for net, c in zip(nets, cells))) * steps
     sample = plan.download(0)
     return {"workload": f"{n} networks, G~U{{5..64}}, cells logU[256,8192], population (growth 0.05, V_div 2, "
-                        f"binomial partition, Moran), telegraph, T={T}, dt={dt}, one hipGraph",
+                        f"binomial partition, Moran), telegraph, T={T}, dt={dt}, "
+                        f"{'direct launches' if a.no_graph else 'one hipGraph'}",
             "networks": n, "total_cells": int(sum(cells)), "mean_G": float(np.mean([x.G for x in nets])),
             "cell_gene_steps": cgs, "cgs_per_s": cgs * a.steps / el, "s_per_run": el / a.steps,
             "graph_kernel_ms": float(np.mean(kms)), "plan_build_s": t_build, "graph_capture_s": t_graph,
@@ -627,6 +629,7 @@ def main() -> int:
                     help="g256: comma-separated forms")
     ap.add_argument("--networks", type=int, default=4096, help="batch: number of networks")
     ap.add_argument("--T", type=float, default=None, help="batch: horizon")
+    ap.add_argument("--no-graph", action="store_true", help="batch: launch the plan's kernels directly, no hipGraph")
     ap.add_argument("--qps", type=float, default=4.0, help="serve: arrival rate")
     ap.add_argument("--requests", type=int, default=60, help="serve: number of requests")
     ap.add_argument("--bytes", type=int, default=1 << 30, help="copy: bytes per copy")

@@ -123,7 +123,22 @@ int validate_inputs(const scrna_network* net, const scrna_kinetics* kin, const s
     if (!net || !kin || !run) return set_error(SCRNA_ERR_INVALID, "null network, kinetics or run");
     const int G = net->G;
     if (G <= 0) return set_error(SCRNA_ERR_INVALID, "G must be positive (got %d)", G);
-    if (G > kMaxGenes) return set_error(SCRNA_ERR_UNSUPPORTED, "G=%d exceeds the supported maximum %d", G, kMaxGenes);
+    if (G > kMaxGenes) {
+        // Past the fused steppers' limit the CLE runs on the reference-structured FP32 path
+        // (rocBLAS products for any G, like ext:93-94), which has no telegraph promoters,
+        // population dynamics or injected noise.
+        bool tele = false;
+        if (net->k_on)
+            for (int g = 0; g < G; ++g) tele |= std::isfinite(net->k_on[g]) || std::isfinite(net->k_off[g]);
+        if (G > kMaxGenesUnfused || run->algorithm != SCRNA_ALG_CLE ||
+            (run->precision != SCRNA_PREC_FP32 && run->precision != SCRNA_PREC_FP32_UNFUSED) || tele ||
+            (pop && pop->enabled) || run->injected_noise)
+            return set_error(SCRNA_ERR_UNSUPPORTED,
+                             "G=%d exceeds the fused steppers' maximum %d: it runs as the FP32 CLE on the "
+                             "reference-structured path (precision fp32 or fp32_unfused, G <= %d) without telegraph "
+                             "promoters, population dynamics or injected noise",
+                             G, kMaxGenes, kMaxGenesUnfused);
+    }
     if (!net->basals || !net->A) return set_error(SCRNA_ERR_INVALID, "basals and A are required");
     for (int i = 0; i < G; ++i) {
         if (!std::isfinite(net->basals[i])) return set_error(SCRNA_ERR_INVALID, "basal %d is not finite", i);
@@ -629,6 +644,7 @@ void plan_release(scrna_plan* p) {
 // cost grows with the in-degree rather than with G.
 static int effective_precision(const scrna_network& net, const scrna_run& run, bool population) {
     if (run.precision == SCRNA_PREC_FP32_VALU) return SCRNA_PREC_FP32;
+    if (run.precision == SCRNA_PREC_FP32 && net.G > kMaxGenes) return SCRNA_PREC_FP32_UNFUSED;
     if (run.precision != SCRNA_PREC_FP32 || run.algorithm != SCRNA_ALG_CLE || population || net.G < 128)
         return run.precision;
     int64_t nnz = 0;
@@ -674,9 +690,11 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
         // Arena plans run on one stream with no events, and hold no
         // population buffers or library handles of their own.
         for (int i = 0; i < n; ++i)
-            if ((pops && pops[i].enabled) || runs[i].precision == SCRNA_PREC_FP32_UNFUSED)
+            if ((pops && pops[i].enabled) ||
+                effective_precision(nets[i], runs[i], false) == SCRNA_PREC_FP32_UNFUSED)
                 return set_error(SCRNA_ERR_UNSUPPORTED,
-                                 "population runs and the reference-structured FP32 path allocate their own buffers");
+                                 "population runs and the reference-structured FP32 path (also every network with "
+                                 "G > %d) allocate their own buffers", kMaxGenes);
     }
     P->nets.resize(n);
     std::vector<uint32_t> blobs;

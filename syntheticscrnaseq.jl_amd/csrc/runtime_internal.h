@@ -34,7 +34,8 @@ struct scrna_ctx {
 
 namespace scrna {
 
-constexpr int kMaxGenes = 256;
+constexpr int kMaxGenes = 256;            // fused steppers (registers / LDS hold a network's genes)
+constexpr int kMaxGenesUnfused = 16384;    // reference-structured FP32 path (rocBLAS products, any G)
 constexpr uint64_t kSnapPoolBit = 1ull << 63;
 
 struct CompiledNet {

@@ -574,6 +574,8 @@ int scrna_submit(scrna_server* S, const scrna_request* req, int64_t* ticket) {
     const scrna_capture* cap = req->capture.enabled ? &req->capture : nullptr;
     int rc = validate_inputs(&req->net, &req->kin, &req->run, nullptr, cap);
     if (rc) return rc;
+    if (req->net.G > kMaxGenes)
+        return set_error(SCRNA_ERR_UNSUPPORTED, "served requests run the fused FP32 stepper: G must be <= %d", kMaxGenes);
     if ((req->run.precision != SCRNA_PREC_FP32 && req->run.precision != SCRNA_PREC_FP32_VALU) ||
         req->run.snapshot_every != 0 || req->run.injected_noise ||
         req->run.algorithm != SCRNA_ALG_CLE)
