@@ -142,7 +142,9 @@ typedef struct scrna_run {
 /* Outputs of one network (all optional except y).
  *  y:          readout, cells x G (x2G for BOTH) column-major, leading dim ld
  *  counts:     captured counts, cells x G column-major int32, leading dim ld
- *  snapshots:  [n_snap][2][G][cell_num] float (species 0 = mRNA, 1 = protein)
+ *  snapshots:  [n_snap][2][G][cell_num] float (species 0 = mRNA, 1 = protein);
+ *              tau-leap counts in snapshots are exact up to 2^24, y, counts and
+ *              moments carry them exactly (int32 state) up to 2^31 - 1
  *  moments:    [n_snap_or_1][G][5] double sums over cells: m, m^2, p, p^2, m*p
  *              (final state when there are no snapshots)
  *  volumes:    [cell_num] final cell volumes (population runs) */

@@ -15,7 +15,8 @@ struct CaptureJob {
     int64_t src_ld, dst_ld;
     int32_t cells, G;
     double mu_log, sigma_log;   // log(eff) - sigma^2/2, sigma (capture.jl:30-31)
-    uint32_t k0, k1, net_id, pad;
+    uint32_t k0, k1, net_id;
+    uint32_t src_int;           // 1: the source holds int32 counts (tau-leap state), 0: floating point
 };
 
 // One moment job = one (m, p) pair of G x cells float matrices.
@@ -23,6 +24,7 @@ struct MomentJob {
     uint64_t m_off, p_off;      // element offsets into the float pool
     uint64_t out_off;           // element offset into the double output ([G][5])
     int32_t cells, G;
+    int32_t src_int, pad;       // src_int 1: the rows hold int32 counts (tau-leap state)
 };
 
 // Segmented schedule of the FP32 stepper: a launch over n_tiles tiles with

@@ -558,7 +558,8 @@ capture_kernel(const CaptureJob* __restrict__ jobs, const T* __restrict__ src_po
     const double z = sqrt(-2.0 * log(U1)) * cos(2.0 * 3.141592653589793 * U2);
     double pc = exp(J.mu_log + J.sigma_log * z);        // capture.jl:36
     pc = pc < 0.0 ? 0.0 : (pc > 1.0 ? 1.0 : pc);
-    const double xv = (double)src_pool[J.src_off + (size_t)g * J.src_ld + c];
+    const size_t si = J.src_off + (size_t)g * J.src_ld + c;
+    const double xv = J.src_int ? (double)((const int32_t*)src_pool)[si] : (double)src_pool[si];
     int64_t n = (int64_t)rint(xv);                       // round half to even (capture.jl:38)
     if (n < 0) n = 0;
     int64_t y = 0;
@@ -582,7 +583,8 @@ moments_kernel(const MomentJob* __restrict__ jobs, const float* __restrict__ poo
     const float* __restrict__ prow = pool + J.p_off + (size_t)g * J.cells;
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0;
     for (int64_t c = threadIdx.x; c < J.cells; c += blockDim.x) {
-        const double a = (double)mrow[c], b = (double)prow[c];
+        const double a = J.src_int ? (double)((const int32_t*)mrow)[c] : (double)mrow[c];
+        const double b = J.src_int ? (double)((const int32_t*)prow)[c] : (double)prow[c];
         s0 += a;
         s1 += a * a;
         s2 += b;
@@ -651,13 +653,14 @@ pop_divide_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ t
     const uint32_t mode = B[H_POPMODE];
     const uint32_t k0 = B[H_SEED_LO], k1 = B[H_SEED_HI];
     const uint32_t net = B[H_NETID];
+    const bool counts = B[H_ALG] == 1u;   // binomial tau-leap (scrna_algorithm 1): int32 counts in the pool
+    int32_t* __restrict__ state_i = (int32_t*)state_pool;
+    int32_t* __restrict__ dstate_i = (int32_t*)dstate;
     for (int g = 0; g < G; ++g) {
         for (int sp = 0; sp < 2; ++sp) {
             const size_t i = off_state + (size_t)(sp * G + g) * C + cell;
-            const float x = state_pool[i];
-            float mother, daughter;
             if (mode & 1u) {
-                int64_t n = (int64_t)rint((double)x);
+                int64_t n = counts ? (int64_t)state_i[i] : (int64_t)rint((double)state_pool[i]);
                 if (n < 0) n = 0;
                 int64_t d = 0;
                 if (n > 0) {
@@ -665,13 +668,18 @@ pop_divide_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ t
                                (net << 4) | kPartition, k0, k1);
                     d = binomial_sample(n, 0.5, st);
                 }
-                mother = (float)(n - d);
-                daughter = (float)d;
+                if (counts) {
+                    state_i[i] = (int32_t)(n - d);
+                    dstate_i[i] = (int32_t)d;
+                } else {
+                    state_pool[i] = (float)(n - d);
+                    dstate[i] = (float)d;
+                }
             } else {
-                mother = daughter = x / 2.0f;
+                const float x = state_pool[i];
+                state_pool[i] = x / 2.0f;
+                dstate[i] = x / 2.0f;
             }
-            state_pool[i] = mother;
-            dstate[i] = daughter;
         }
         dprom[off_prom + (size_t)g * C + cell] = prom_pool[off_prom + (size_t)g * C + cell];
     }
@@ -708,9 +716,12 @@ pop_place_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
     const int w = claim[off_vol + r];
     if (w == 0) return;
     const int mother = w - 1;
+    // Raw 32-bit words: FP32 molecules of the CLE, int32 counts of the tau-leap.
+    uint32_t* __restrict__ sw = (uint32_t*)state_pool;
+    const uint32_t* __restrict__ dw = (const uint32_t*)dstate;
     for (int g = 0; g < G; ++g) {
-        state_pool[off_state + (size_t)g * C + r] = dstate[off_state + (size_t)g * C + mother];
-        state_pool[off_state + (size_t)(G + g) * C + r] = dstate[off_state + (size_t)(G + g) * C + mother];
+        sw[off_state + (size_t)g * C + r] = dw[off_state + (size_t)g * C + mother];
+        sw[off_state + (size_t)(G + g) * C + r] = dw[off_state + (size_t)(G + g) * C + mother];
         prom_pool[off_prom + (size_t)g * C + r] = dprom[off_prom + (size_t)g * C + mother];
     }
     vol_pool[off_vol + r] = dvol[off_vol + mother];

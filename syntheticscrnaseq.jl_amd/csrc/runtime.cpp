@@ -873,6 +873,7 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
             j.k0 = (uint32_t)cn.seed;
             j.k1 = (uint32_t)(cn.seed >> 32);
             j.net_id = cn.net_id;
+            j.src_int = cn.algorithm == SCRNA_ALG_BINOMIAL_TAU_LEAP ? 1u : 0u;
             cjobs.push_back(j);
             P->max_cap_elems = std::max<int64_t>(P->max_cap_elems, (int64_t)GC);
         }
@@ -886,6 +887,7 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
             } else {
                 j.m_off = cn.off_state;
                 j.p_off = cn.off_state + GC;
+                j.src_int = cn.algorithm == SCRNA_ALG_BINOMIAL_TAU_LEAP ? 1 : 0;   // the final state pool
             }
             j.out_off = cn.off_mom + (size_t)k * cn.G * 5;
             j.cells = (int32_t)cn.cells;
@@ -1185,6 +1187,7 @@ int plan_download(scrna_plan* P, int32_t i, scrna_output* out) {
         if (ld < (int64_t)C) return set_error(SCRNA_ERR_INVALID, "output leading dimension %lld < cells", (long long)ld);
         std::vector<float> h(2 * GC);
         HIP_TRY(hipMemcpy(h.data(), P->d_state + cn.off_state, 2 * GC * sizeof(float), hipMemcpyDeviceToHost));
+        const bool counts = cn.algorithm == SCRNA_ALG_BINOMIAL_TAU_LEAP;   // int32 counts in the pool
         // cle.jl:123-129 -- columns are genes; :both puts mRNA first.
         const int ncol = cn.readout == SCRNA_READOUT_BOTH ? 2 * (int)G : (int)G;
         for (int col = 0; col < ncol; ++col) {
@@ -1194,7 +1197,12 @@ int plan_download(scrna_plan* P, int32_t i, scrna_output* out) {
             else src_row = col;   // both: rows 0..G-1 mRNA, G..2G-1 protein
             const float* s = h.data() + src_row * C;
             double* d = out->y + (size_t)col * ld;
-            for (size_t c = 0; c < C; ++c) d[c] = (double)s[c];
+            if (counts) {
+                const int32_t* si = (const int32_t*)s;
+                for (size_t c = 0; c < C; ++c) d[c] = (double)si[c];
+            } else {
+                for (size_t c = 0; c < C; ++c) d[c] = (double)s[c];
+            }
         }
     }
     if (out->counts) {

@@ -150,8 +150,8 @@ tauleap_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restric
         const uint32_t tf = telew[g * 8 + T_FLAGS];
         sw_mask |= (tf & 1u) << k;
         if (resume) {
-            M(k) = (int)state_pool[off_state + (size_t)g * C + cell];
-            P(k) = (int)state_pool[off_state + (size_t)(G + g) * C + cell];
+            M(k) = ((const int32_t*)state_pool)[off_state + (size_t)g * C + cell];     // int32 counts
+            P(k) = ((const int32_t*)state_pool)[off_state + (size_t)(G + g) * C + cell];
             on |= (uint32_t)(prom_pool[off_prom + (size_t)g * C + cell] != 0) << k;
         } else {
             M(k) = (int)init_d[2 * g];                 // round(Int, steady state), tauleap_binomial.jl:61-62
@@ -275,8 +275,8 @@ tauleap_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restric
 #pragma unroll 1
         for (int k = 0; k < ng; ++k) {
             const int g = g0 + k;
-            state_pool[off_state + (size_t)g * C + cell] = (float)M(k);
-            state_pool[off_state + (size_t)(G + g) * C + cell] = (float)P(k);
+            ((int32_t*)state_pool)[off_state + (size_t)g * C + cell] = M(k);         // exact past 2^24
+            ((int32_t*)state_pool)[off_state + (size_t)(G + g) * C + cell] = P(k);
             prom_pool[off_prom + (size_t)g * C + cell] = (uint8_t)((on >> k) & 1u);
         }
         if (pop && wave == 0) vol_pool[off_vol + cell] = V;

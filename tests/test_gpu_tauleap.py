@@ -204,3 +204,33 @@ def test_population_growth_noise_matches_oracle(ctx, partition_check):
         net, KIN, sc.PopulationConfig(cell_num=400, growth_rate=0.05, V_div=2.0, V_init=(0.8, 1.2),
                                       div_check_interval=partition_check), dt=dt, T=T, cell_num=400,
         seed=seed)["V"])
+
+
+BIG_NET = sc.GeneNetwork(3, np.array([4.0, 2.0, 3.0]), np.array([[0, 0.5, 0], [0, 0, -0.5], [0.3, 0, 0]]))
+BIG_KIN = sc.KineticParams(k_t=2.0e5, K_d=1e7, n=2.0, mu_m=0.02, mu_p=0.2)
+
+
+def test_counts_above_2_24_stay_exact(ctx):
+    """Protein counts near 2e8 (past FP32's 2^24 integer range): the state pool
+    holds tau-leap counts as int32 through segment boundaries, division phases,
+    capture and the final download, so they agree with the oracle exactly."""
+    print("tau-leap counts ~2e8, plain / population / capture", flush=True)
+    ref = orc.tauleap(BIG_NET, BIG_KIN, dt=0.05, T=5.0, cell_num=256, seed=3)
+    assert ref["p"].max() > 2 ** 24 and np.any(ref["p"] % 16 != 0)
+    y = sc.simulate(BIG_NET, sc.BinomialTauLeap(0.05), BIG_KIN, cell_num=256, T=5.0, readout="both", seed=3)
+    assert np.array_equal(y, both(ref))
+
+    pop = sc.PopulationConfig(cell_num=200, growth_rate=0.05, V_div=2.0, V_init=(0.8, 1.2), div_check_interval=5)
+    res = sc.simulate(BIG_NET, sc.BinomialTauLeap(0.1), BIG_KIN, "gpu", pop, T=20.0, readout="both", seed=5,
+                      full=True)
+    rp = orc.tauleap_population(BIG_NET, BIG_KIN, pop, dt=0.1, T=20.0, cell_num=200, seed=5)
+    assert rp["p"].max() > 2 ** 24
+    assert np.array_equal(res.volumes, rp["V"])
+    assert np.array_equal(res.y, both(rp))
+
+    cap = sc.CaptureModel(efficiency=0.3, efficiency_std=0.2, readout="protein")
+    out = sc.simulate_gpu_batch([BIG_NET], sc.BinomialTauLeap(0.05), BIG_KIN, cell_num=[256], T=[5.0],
+                                readout="protein", seed=3, full=True, capture=cap)
+    assert np.array_equal(out[0].y, ref["p"].T)
+    want, _ = orc.capture(ref["p"], 0.3, 0.2, 3, net_id=0)
+    assert np.array_equal(out[0].counts, want.T)
