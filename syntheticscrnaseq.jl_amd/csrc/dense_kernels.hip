@@ -18,17 +18,21 @@
 // sources' Hill values.
 //
 // Work mapping.  A workgroup owns 16 cells and NW = Gp/(16*NB) waves (Gp = G
-// rounded up to 32); wave w owns NB blocks of 16 output genes, block b being
-// genes 16*(w*NB+b) + [0,16).  In a 16x16 MFMA result lane l holds gene
-// 16*(w*NB+b) + (l & 15) for the 4 cells 4*(l>>4) + v, v = 0..3, so those
-// cells' (m, p) stay in the lane's VGPRs for the whole horizon, in
-// accumulator layout.  W^T for the wave's output genes is register-resident
-// as the B operand (4 VGPRs per 32-wide K chunk, gene block and bf16 plane).
-// NB = 2 halves the waves per workgroup, which gives each wave the 256-VGPR
-// budget of 2 waves per SIMD that BF16X3's two weight planes need.  Each step
-// every lane writes act of its genes and cells into an LDS tile laid out
-// [cell][gene] -- the A operand's layout -- the workgroup synchronises once
-// (double-buffered tile), and each wave issues NB*Gp/32 MFMAs (x3 for BF16X3).
+// rounded up to 32); wave w owns NB blocks of 16 genes, block b being genes
+// 16*(w*NB+b) + [0,16).  The contraction runs transposed, reg^T = W act^T:
+// W is the MFMA's A operand and the act tile its B operand, so in the 16x16
+// result lane l holds genes 16*(w*NB+b) + 4*(l>>4) + v, v = 0..3, of the
+// cell l & 15, and those four genes' (m, p) stay in the lane's VGPRs for the
+// whole horizon, in accumulator layout.  W for the wave's genes is
+// register-resident (4 VGPRs per 32-wide K chunk, gene block and bf16 plane;
+// a lane holds row l & 15 of the block, sources 8*(l>>4) + [0,8) of each
+// chunk).  NB = 2 halves the waves per workgroup, which gives each wave the
+// 256-VGPR budget of 2 waves per SIMD that BF16X3's two weight planes need.
+// Each step every lane writes act of its four genes for its cell into an LDS
+// tile laid out [cell][gene] (one 8-byte store per bf16 plane), the
+// workgroup synchronises once (double-buffered tile), and each wave issues
+// NB*Gp/32 MFMAs (x3 for BF16X3).  A lane's genes come in pairs (g, g+1),
+// g even, which is exactly the gene pair of one normal block (cell, s, g>>1).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -53,12 +57,6 @@ __device__ __forceinline__ uint32_t pick4(const U4& u, int i) {
 }
 
 constexpr int kDenseMaxKC = 8;   // Gp <= 256
-
-// Cross-lane move inside a quad of lanes as a DPP operand of a v_mov (no
-// LDS round trip as with ds_bpermute): lane j receives lane j ^ 1's value.
-__device__ __forceinline__ uint32_t dpp_xor1(uint32_t x) {
-    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
-}
 
 // CLAMP keeps max(m, 0), max(p, 0) and the protein variance's max(., 0) of
 // the update, which only a network starting from negative counts
@@ -104,12 +102,12 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
 
     const int l = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
-    const int rg = l >> 4;                 // row group: cells 4*rg .. 4*rg+3 of the tile
-    const int par = l & 1;
+    const int cl = l & 15;                 // the lane's cell: row cl of the tile
+    const int qd = l >> 4;                 // its gene quad: genes 4*qd .. 4*qd+3 of each block
 
     // LDS: act tiles [plane][buffer][16 rows][Gp + 8] bf16 (FP32: [buffer][16 rows][Gp + 4] f32, rows
     // 16 bytes apart in bank order so that the 16-byte reads of 16 lanes hit distinct banks), then FP32
-    // gate Hill values.
+    // gate Hill values [buffer][2 S][16 cells].
     constexpr bool F32 = MODE == 3;
     const int rowstride = F32 ? Gp + 4 : Gp + 8;
     const int tile_elems = 16 * rowstride;
@@ -123,34 +121,33 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
     const uint32_t* __restrict__ gates = B + B[H_OFF_GATES];
     const uint32_t* __restrict__ telew = B + B[H_OFF_TELE];
 
-    // Per gene block: weights, constants, state.  A gene's gate, when it has
+    // Per gene (b, v): weights, constants, state.  A gene's gate, when it has
     // exactly one and it has two sources (the form of every gate the prior
-    // draws), lives in registers: strength*beta, OR flag and the offsets of
-    // its sources' Hill rows (AND reads act rows, OR rep rows).  Other gate
-    // layouts take the loop over the blob's records (gslow).
+    // draws), lives in registers as a signed weight and the LDS offsets of its
+    // sources' Hill rows (AND reads act rows, OR rep rows): an AND gate adds
+    // w*act_a*act_b, an OR gate w*(1 - rep_a*rep_b) = w - w*rep_a*rep_b, whose
+    // constant w joins the bias.  Other gate layouts take the loop over the
+    // blob's records (gslow).
     bf16x8 whi[NB][F32 ? 1 : kDenseMaxKC], wlo[NB][F32 ? 1 : kDenseMaxKC];
     float wf[NB][F32 ? 4 * kDenseMaxKC * 2 : 1];      // FP32: W[16 blk + (l & 15)][(l >> 4) KT + t]
-    int g[NB], slot[NB], q0[NB], q1[NB];
-    bool gl[NB];
-    float gw[NB];
-    int ga[NB], gb[NB];
-    bool gor[NB], gslow[NB], wave_gate[NB], wave_slow[NB];
-    float bias[NB];                        // basal rate + rowsum(A_neg) bias: the drive's constant part
-    uint32_t tflags[NB];
-    uint32_t thr_on[NB], thr_off[NB];     // floor(threshold / 2^14), see cle_switch
+    int slot[NB][4];
+    float gw[NB][4];                       // signed gate weight (0: no register gate)
+    uint32_t gab[NB][4];                   // LDS float offsets of the two source rows, 16 bits each
+    uint32_t gslow_m[NB];                  // bit v: gene v takes the record loop
+    bool wave_gate[NB], wave_slow[NB];
+    f32x4 bias[NB];                        // basal rate + rowsum(A_neg) (+ OR weights): the MFMA's start value
+    uint32_t tflags[NB][4];
+    uint32_t thr_on[NB][4], thr_off[NB][4];   // floor(threshold / 2^14), see cle_switch
     float m[NB][4], p[NB][4];
     uint32_t on = 0;                       // promoter bit 4*b + v
 
-    auto row = [&](int v) { return 4 * rg + v; };
-    auto live = [&](int v) { return row(v) < tile.ncell; };
-    auto cell = [&](int v) { return tile.cell0 + (live(v) ? row(v) : 0); };
+    const bool live = cl < tile.ncell;
+    const int cell = tile.cell0 + (live ? cl : 0);
+    auto gene = [&](int b, int v) { return 16 * (w * NB + b) + 4 * qd + v; };
 
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         const int blk = w * NB + b;
-        g[b] = blk * 16 + (l & 15);
-        gl[b] = g[b] < G;
-        const int gc = gl[b] ? g[b] : 0;
         if constexpr (F32) {
             const float* __restrict__ wp = (const float*)wpool + off_w * 4;
 #pragma unroll
@@ -165,58 +162,65 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
                 }
             }
         }
-        // Padding genes (g >= G) get beta = bias = 0, zero weight rows and
-        // columns, an OFF promoter and zero state, so every step leaves them at
-        // exactly 0 and the update needs no mask.
-        bias[b] = gl[b] ? d_as_f(B[B[H_OFF_BIAS] + gc]) + d_as_f(B[B[H_OFF_BETA] + gc]) : 0.0f;
-        slot[b] = gl[b] ? ((const int32_t*)(B + B[H_OFF_SLOT]))[g[b]] : -1;
-        tflags[b] = gl[b] ? telew[g[b] * 8 + T_FLAGS] : 0u;
-        thr_on[b] = telew[gc * 8 + T_ON_HI] != 0u ? 1u << 18 : telew[gc * 8 + T_ON_LO] >> 14;
-        thr_off[b] = telew[gc * 8 + T_OFF_HI] != 0u ? 1u << 18 : telew[gc * 8 + T_OFF_LO] >> 14;
-        q0[b] = gl[b] ? gate_off[g[b]] : 0;
-        q1[b] = gl[b] ? gate_off[g[b] + 1] : 0;
-        gw[b] = 0.0f;
-        ga[b] = gb[b] = 0;
-        gor[b] = gslow[b] = false;
-        bool greg = false;
-        if (q1[b] > q0[b]) {
-            const uint32_t hd = gates[q0[b]];
-            const int type = (int)(hd >> 24), ns = (int)(hd & 0xffffu);
-            if (ns == 2 && q1[b] - q0[b] == 2 + ns) {
-                greg = true;
-                gw[b] = d_as_f(gates[q0[b] + 1]);
-                gor[b] = type == (int)GATE_OR;
-                ga[b] = (2 * (int)gates[q0[b] + 2] + type) * 16;
-                gb[b] = (2 * (int)gates[q0[b] + 3] + type) * 16;
-            } else {
-                gslow[b] = true;
-            }
-        }
-        wave_gate[b] = __builtin_amdgcn_ballot_w64(greg) != 0;
-        wave_slow[b] = __builtin_amdgcn_ballot_w64(gslow[b]) != 0;
+        bool any_gate = false;
+        gslow_m[b] = 0u;
+        U4 uinit = U4{0u, 0u, 0u, 0u};
+        if (!resume && tele)   // the promoter-init block (cell, 0, g>>2) covers the lane's four genes
+            uinit = philox4x32_10((uint32_t)cell, 0u, (uint32_t)(gene(b, 0) >> 2), (B[H_NETID] << 4) | kPromInit,
+                                  k0, k1);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
+            const int g = gene(b, v);
+            const bool gl = g < G;
+            const int gc = gl ? g : 0;
+            // Padding genes (g >= G) get beta = bias = 0, zero weight rows and
+            // columns, an OFF promoter and zero state, so every step leaves them at
+            // exactly 0 and the update needs no mask.
+            float bv = gl ? d_as_f(B[B[H_OFF_BIAS] + gc]) + d_as_f(B[B[H_OFF_BETA] + gc]) : 0.0f;
+            slot[b][v] = gl ? ((const int32_t*)(B + B[H_OFF_SLOT]))[g] : -1;
+            tflags[b][v] = gl ? telew[g * 8 + T_FLAGS] : 0u;
+            thr_on[b][v] = telew[gc * 8 + T_ON_HI] != 0u ? 1u << 18 : telew[gc * 8 + T_ON_LO] >> 14;
+            thr_off[b][v] = telew[gc * 8 + T_OFF_HI] != 0u ? 1u << 18 : telew[gc * 8 + T_OFF_LO] >> 14;
+            const int q0 = gl ? gate_off[g] : 0;
+            const int q1 = gl ? gate_off[g + 1] : 0;
+            gw[b][v] = 0.0f;
+            gab[b][v] = 0u;
+            if (q1 > q0) {
+                const uint32_t hd = gates[q0];
+                const int type = (int)(hd >> 24), ns = (int)(hd & 0xffffu);
+                if (ns == 2 && q1 - q0 == 2 + ns) {
+                    any_gate = true;
+                    const float gwt = d_as_f(gates[q0 + 1]);
+                    const bool is_or = type == (int)GATE_OR;
+                    gw[b][v] = is_or ? -gwt : gwt;
+                    if (is_or) bv += gwt;
+                    gab[b][v] = (uint32_t)((2 * (int)gates[q0 + 2] + type) * 16) |
+                                ((uint32_t)((2 * (int)gates[q0 + 3] + type) * 16) << 16);
+                } else {
+                    gslow_m[b] |= 1u << v;
+                }
+            }
+            bias[b][v] = bv;
             m[b][v] = 0.0f;
             p[b][v] = 0.0f;
-            if (gl[b]) {
+            if (gl) {
                 if (resume) {
-                    m[b][v] = state_pool[off_state + (size_t)g[b] * C + cell(v)];
-                    p[b][v] = state_pool[off_state + (size_t)(G + g[b]) * C + cell(v)];
-                    on |= (uint32_t)(prom_pool[off_prom + (size_t)g[b] * C + cell(v)] != 0) << (4 * b + v);
+                    m[b][v] = state_pool[off_state + (size_t)g * C + cell];
+                    p[b][v] = state_pool[off_state + (size_t)(G + g) * C + cell];
+                    on |= (uint32_t)(prom_pool[off_prom + (size_t)g * C + cell] != 0) << (4 * b + v);
                 } else {
                     const uint32_t* initw = B + B[H_OFF_INIT];
-                    m[b][v] = d_as_f(initw[2 * g[b]]);
-                    p[b][v] = d_as_f(initw[2 * g[b] + 1]);
+                    m[b][v] = d_as_f(initw[2 * g]);
+                    p[b][v] = d_as_f(initw[2 * g + 1]);
                     bool o = true;
-                    if (tele && (tflags[b] & 2u)) {
-                        const U4 u = philox4x32_10((uint32_t)cell(v), 0u, (uint32_t)(g[b] >> 2),
-                                                   (B[H_NETID] << 4) | kPromInit, k0, k1);
-                        o = (uint64_t)pick4(u, g[b]) < d_u64(telew, g[b] * 8 + T_INIT_LO);
-                    }
+                    if (tele && (tflags[b][v] & 2u))   // tauleap_binomial.jl:64-76
+                        o = (uint64_t)pick4(uinit, g) < d_u64(telew, g * 8 + T_INIT_LO);
                     on |= (uint32_t)o << (4 * b + v);
                 }
             }
         }
+        wave_gate[b] = __builtin_amdgcn_ballot_w64(any_gate) != 0;
+        wave_slow[b] = __builtin_amdgcn_ballot_w64(gslow_m[b] != 0u) != 0;
     }
 
     const int s_stop = min(min(s_end, tile.s1), n_steps);
@@ -231,9 +235,11 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
 
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
-            // Hill values of the lane's gene for its 4 cells -> LDS.  The n = 2
-            // test sits outside the cell loop so that the n = 2 path never
-            // evaluates log/exp (as a select, both sides would run).
+            // Hill values of the lane's four genes for its cell -> LDS, one
+            // 8-byte (BF16) or 16-byte (FP32) store per plane.  The n = 2 test
+            // sits outside the gene loop so that the n = 2 path never evaluates
+            // log/exp (as a select, both sides would run).
+            float act[4], rep[4];
             auto hill_phase = [&](auto h2) {
                 constexpr bool H2 = decltype(h2)::value;
 #pragma unroll
@@ -242,65 +248,106 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
                     const float x = p[b][v] * invK;
                     float xn = H2 ? x * x : __builtin_amdgcn_exp2f(hill_n * __builtin_amdgcn_logf(x));
                     xn = fminf(xn, 1e30f);
-                    const float rep = __builtin_amdgcn_rcpf(1.0f + xn);
-                    const float act = xn * rep;
-                    if constexpr (F32) {
-                        TF[row(v) * rowstride + g[b]] = act;
-                    } else {
-                        const __bf16 hi = (__bf16)act;
-                        T0[row(v) * rowstride + g[b]] = hi;
-                        if (MODE == 2) T1[row(v) * rowstride + g[b]] = (__bf16)(act - (float)hi);
-                    }
-                    if (slot[b] >= 0) {
-                        GH[(2 * slot[b]) * 16 + row(v)] = act;
-                        GH[(2 * slot[b] + 1) * 16 + row(v)] = rep;
-                    }
+                    rep[v] = __builtin_amdgcn_rcpf(1.0f + xn);
+                    act[v] = xn * rep[v];
                 }
             };
             if (hill2)
                 hill_phase(std::true_type{});
             else
                 hill_phase(std::false_type{});
+            const int col = 16 * (w * NB + b) + 4 * qd;
+            if constexpr (F32) {
+                *(f32x4*)(TF + cl * rowstride + col) = f32x4{act[0], act[1], act[2], act[3]};
+            } else {
+                typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+                bf16x4 hi;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) hi[v] = (__bf16)act[v];
+                *(bf16x4*)(T0 + cl * rowstride + col) = hi;
+                if (MODE == 2) {
+                    bf16x4 lo;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) lo[v] = (__bf16)(act[v] - (float)hi[v]);
+                    *(bf16x4*)(T1 + cl * rowstride + col) = lo;
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                if (slot[b][v] >= 0) {
+                    GH[(2 * slot[b][v]) * 16 + cl] = act[v];
+                    GH[(2 * slot[b][v] + 1) * 16 + cl] = rep[v];
+                }
+            }
         }
         __syncthreads();
 
-        // reg = act * W^T on the matrix cores.
-        f32x4 acc[NB];
+        // The step's normals: the block (cell, s, g>>1) carries genes g (words
+        // 0,1) and g+1 (words 2,3) for even g, two blocks per gene block.
+        // With one gene block per wave they are drawn here, between the MFMA
+        // issue and the first use of its result; with two (BF16X3's register
+        // budget) each pair is drawn just before its update.
+        constexpr bool EARLY = NB == 1 && !INJECT;
+        float nr2[NB][4], ncs[NB][4], nsn[NB][4];
+        uint32_t nwa[NB][4], nwb[NB][4];
+        auto draw = [&](const int b, const int h) {
+            const U4 u = philox4x32_10_uk((uint32_t)cell, (uint32_t)s, (uint32_t)(gene(b, 2 * h) >> 1), c3n, k0, k1);
+            normal_pair_r2(u.x, u.y, neg2ln2dt, nr2[b][2 * h], ncs[b][2 * h], nsn[b][2 * h]);
+            normal_pair_r2(u.z, u.w, neg2ln2dt, nr2[b][2 * h + 1], ncs[b][2 * h + 1], nsn[b][2 * h + 1]);
+            nwa[b][2 * h] = u.x;
+            nwb[b][2 * h] = u.y;
+            nwa[b][2 * h + 1] = u.z;
+            nwb[b][2 * h + 1] = u.w;
+        };
+        auto draw_early = [&]() {
+            if constexpr (EARLY) {
 #pragma unroll
-        for (int b = 0; b < NB; ++b) acc[b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                for (int b = 0; b < NB; ++b) {
+                    draw(b, 0);
+                    draw(b, 1);
+                }
+            }
+        };
+
+        // reg^T = W * act^T on the matrix cores: W (register-resident) is the
+        // A operand, the act tile the B operand, and the bias the start value,
+        // so lane l receives genes 4*(l >> 4) + v of its block for cell l & 15.
+        // The BF16 forms accumulate one chain per gene block: the normals drawn
+        // between the MFMA issue and the result's first use (and, with two
+        // blocks, the other block's chain) cover the chain's latency, and no
+        // partial sums are added up afterwards.  The FP32 form, whose 16x16x4
+        // MFMAs are 8 times as many, keeps two chains (even and odd groups of
+        // four), summed after the normals are drawn.
+        f32x4 acc[NB], acc2[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            acc[b] = bias[b];
+            acc2[b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        }
         if constexpr (F32) {
-            // Two accumulation chains per block, summed after the loop.
-            f32x4 acc2[NB];
-#pragma unroll
-            for (int b = 0; b < NB; ++b) acc2[b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-            const float* arow = TF + (l & 15) * rowstride + rg * KT;
+            const float* arow = TF + cl * rowstride + qd * KT;
 #pragma unroll
             for (int t = 0; t < 8 * kDenseMaxKC; t += 4) {
                 if (t < KT) {
                     const f32x4 a4 = *(const f32x4*)(arow + t);
 #pragma unroll
                     for (int b = 0; b < NB; ++b) {
-                        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[0], wf[b][t], acc[b], 0, 0, 0);
-                        acc2[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[1], wf[b][t + 1], acc2[b], 0, 0, 0);
-                        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[2], wf[b][t + 2], acc[b], 0, 0, 0);
-                        acc2[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[3], wf[b][t + 3], acc2[b], 0, 0, 0);
+                        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(wf[b][t], a4[0], acc[b], 0, 0, 0);
+                        acc2[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(wf[b][t + 1], a4[1], acc2[b], 0, 0, 0);
+                        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(wf[b][t + 2], a4[2], acc[b], 0, 0, 0);
+                        acc2[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(wf[b][t + 3], a4[3], acc2[b], 0, 0, 0);
                     }
                 }
             }
-#pragma unroll
-            for (int b = 0; b < NB; ++b) acc[b] += acc2[b];
+            draw_early();
         }
         if constexpr (!F32) {
-            // With the chunk count a compile-time constant, the A reads of all
-            // chunks issue ahead of the MFMAs, and even and odd chunks
-            // accumulate in two independent chains.
-            const __bf16* arow0 = T0 + (l & 15) * rowstride + 8 * rg;
-            const __bf16* arow1 = T1 + (l & 15) * rowstride + 8 * rg;
+            // With the chunk count a compile-time constant, the act reads of all
+            // chunks issue ahead of the MFMAs.
+            const __bf16* arow0 = T0 + cl * rowstride + 8 * qd;
+            const __bf16* arow1 = T1 + cl * rowstride + 8 * qd;
             auto contract = [&](auto kcn) {
                 constexpr int KCN = decltype(kcn)::value;
-                f32x4 acc2[NB];
-#pragma unroll
-                for (int b = 0; b < NB; ++b) acc2[b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
                 for (int kc = 0; kc < KCN; ++kc) {
                     const bf16x8 a = *(const bf16x8*)(arow0 + 32 * kc);
@@ -308,16 +355,15 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
                     if (MODE == 2) alo = *(const bf16x8*)(arow1 + 32 * kc);
 #pragma unroll
                     for (int b = 0; b < NB; ++b) {
-                        f32x4& c = (kc & 1) ? acc2[b] : acc[b];
-                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, whi[b][kc], c, 0, 0, 0);
+                        f32x4& c = acc[b];
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(whi[b][kc], a, c, 0, 0, 0);
                         if (MODE == 2) {
-                            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, whi[b][kc], c, 0, 0, 0);
-                            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wlo[b][kc], c, 0, 0, 0);
+                            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(whi[b][kc], alo, c, 0, 0, 0);
+                            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wlo[b][kc], a, c, 0, 0, 0);
                         }
                     }
                 }
-#pragma unroll
-                for (int b = 0; b < NB; ++b) acc[b] += acc2[b];
+                draw_early();   // in the MFMAs' basic block, so that the scheduler can overlap them
             };
             switch (KC) {
                 case 8: contract(std::integral_constant<int, 8>{}); break;
@@ -331,47 +377,54 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
             }
         }
 
-        // Elementwise update of element (b, v) (cle.jl:108-120).
+        if (F32) {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) acc[b] += acc2[b];
+        }
+
+        // Elementwise update of gene (b, v) (cle.jl:108-120).
         // Noise enters as xi_m = c*sqrt(r2), xi_p = sn*sqrt(r2) (Box-Muller radius
         // squared, scaled by dt), so xi*sqrt(var) costs one square root, sqrt(r2*var).
-        // wa, wb: the radius and angle words of element (b, v)'s normal block,
+        // wa, wb: the radius and angle words of the gene's normal block,
         // whose low 9 bits are its switching bits.
         auto update = [&](const int b, const int v, float r2, float c, float sn, uint32_t wa, uint32_t wb) {
-            float reg = acc[b][v] + bias[b];
+            float reg = acc[b][v];
             // AND / OR (cle.jl:89-106).  Lanes without a register gate read a
             // valid Hill row with weight 0.
             if (wave_gate[b]) {
-                const float prod = GH[ga[b] + row(v)] * GH[gb[b] + row(v)];
-                reg = fmaf(gw[b], gor[b] ? 1.0f - prod : prod, reg);
+                const float prod = GH[(gab[b][v] & 0xffffu) + cl] * GH[(gab[b][v] >> 16) + cl];
+                reg = fmaf(gw[b][v], prod, reg);
             }
-            if (wave_slow[b] && gslow[b]) {
-                for (int q = q0[b]; q < q1[b];) {
+            if (wave_slow[b] && ((gslow_m[b] >> v) & 1u)) {
+                const int g = gene(b, v);
+                const int q1 = gate_off[g + 1];
+                for (int q = gate_off[g]; q < q1;) {
                     const uint32_t hd = gates[q];
                     const uint32_t type = hd >> 24;
                     const int ns = (int)(hd & 0xffffu);
                     float prod = 1.0f;
-                    for (int i = 0; i < ns; ++i) prod *= GH[(2 * (int)gates[q + 2 + i] + (int)type) * 16 + row(v)];
+                    for (int i = 0; i < ns; ++i) prod *= GH[(2 * (int)gates[q + 2 + i] + (int)type) * 16 + cl];
                     reg = fmaf(d_as_f(gates[q + 1]), type == GATE_AND ? prod : 1.0f - prod, reg);
                     q += 2 + ns;
                 }
             }
             // Telegraph switching first (tauleap_binomial.jl:110-121), as in
-            // cle_switch with one gene per lane: X = q << 14 | e flips iff below
-            // the threshold of the current state; e (the top 14 bits of the
-            // gene's word of the switch block) only matters when q == T.
+            // cle_switch: X = q << 14 | e flips iff below the threshold of the
+            // current state; e (the top 14 bits of the gene's word of the switch
+            // block) only matters when q == T.
             if (tele) {
                 const int bit = 4 * b + v;
                 const bool cur = (on >> bit) & 1u;
-                const bool sw = tflags[b] & 1u;
+                const bool sw = tflags[b][v] & 1u;
                 const uint32_t q = ((wa & 0x1FFu) << 9) | (wb & 0x1FFu);
-                const uint32_t T = cur ? thr_off[b] : thr_on[b];
+                const uint32_t T = cur ? thr_off[b][v] : thr_on[b][v];
                 bool flip = sw && q < T;
                 const bool amb = sw && q == T;
                 if (__builtin_amdgcn_ballot_w64(amb) != 0) {
-                    const U4 u = philox4x32_10_uk((uint32_t)cell(v), (uint32_t)s, (uint32_t)(g[b] >> 2), c3s, k0,
-                                                  k1);
-                    const uint32_t e = pick4(u, g[b]) >> 18;
-                    const int gc = gl[b] ? g[b] : 0;
+                    const int g = gene(b, v);
+                    const U4 u = philox4x32_10_uk((uint32_t)cell, (uint32_t)s, (uint32_t)(g >> 2), c3s, k0, k1);
+                    const uint32_t e = pick4(u, g) >> 18;
+                    const int gc = g < G ? g : 0;
                     const uint64_t thr = d_u64(telew, gc * 8 + (cur ? T_OFF_LO : T_ON_LO));
                     if (amb) flip = ((((uint64_t)q << 14) | e) < thr);
                 }
@@ -403,39 +456,27 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             if (inject) {
-                const size_t base = ((size_t)s * G + (gl[b] ? g[b] : 0)) * 2;
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
+                    const int g = gene(b, v);
+                    const size_t base = ((size_t)s * G + (g < G ? g : 0)) * 2;
                     // the switching bits still come from the normal block
                     uint32_t wa = 0u, wb = 0u;
                     if (tele) {
-                        const U4 u = philox4x32_10_uk((uint32_t)cell(v), (uint32_t)s, (uint32_t)(g[b] >> 1), c3n, k0,
-                                                      k1);
-                        wa = (g[b] & 1) ? u.z : u.x;
-                        wb = (g[b] & 1) ? u.w : u.y;
+                        const U4 u = philox4x32_10_uk((uint32_t)cell, (uint32_t)s, (uint32_t)(g >> 1), c3n, k0, k1);
+                        wa = (g & 1) ? u.z : u.x;
+                        wb = (g & 1) ? u.w : u.y;
                     }
-                    update(b, v, dt, inj_pool[off_inj + base * C + cell(v)], inj_pool[off_inj + (base + 1) * C + cell(v)],
+                    update(b, v, dt, inj_pool[off_inj + base * C + cell], inj_pool[off_inj + (base + 1) * C + cell],
                            wa, wb);
                 }
             } else {
-                // The normal block (cell, s, g>>1) carries genes g (words 0,1) and
-                // g^1 (words 2,3), i.e. lanes l and l^1, which own the same cells.
-                // The even lane draws element t's block, the odd lane element
-                // t+2's, and each hands its partner the partner's half.
 #pragma unroll
-                for (int t = 0; t < 2; ++t) {
-                    const int cm = par ? cell(t + 2) : cell(t);
-                    const U4 u = philox4x32_10_uk((uint32_t)cm, (uint32_t)s, (uint32_t)(g[b] >> 1), c3n, k0, k1);
-                    const uint32_t mine_a = par ? u.z : u.x, mine_b = par ? u.w : u.y;
-                    const uint32_t got_a = dpp_xor1(par ? u.x : u.z);
-                    const uint32_t got_b = dpp_xor1(par ? u.y : u.w);
-                    float r2, c, sn;
-                    const uint32_t ta = par ? got_a : mine_a, tb = par ? got_b : mine_b;
-                    normal_pair_r2(ta, tb, neg2ln2dt, r2, c, sn);
-                    update(b, t, r2, c, sn, ta, tb);
-                    const uint32_t ua = par ? mine_a : got_a, ub = par ? mine_b : got_b;
-                    normal_pair_r2(ua, ub, neg2ln2dt, r2, c, sn);
-                    update(b, t + 2, r2, c, sn, ua, ub);
+                for (int h = 0; h < 2; ++h) {
+                    if constexpr (!EARLY) draw(b, h);
+#pragma unroll
+                    for (int v = 2 * h; v < 2 * h + 2; ++v)
+                        update(b, v, nr2[b][v], ncs[b][v], nsn[b][v], nwa[b][v], nwb[b][v]);
                 }
             }
         }
@@ -443,29 +484,33 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
         if (s == next_snap) {
             next_snap += snap_every;
             const int snap = (s + 1) / snap_every - 1;
+            if (live) {
 #pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                if (!gl[b]) continue;
+                for (int b = 0; b < NB; ++b) {
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    if (live(v)) {
-                        snap_pool[off_snap + ((size_t)(2 * snap) * G + g[b]) * C + cell(v)] = m[b][v];
-                        snap_pool[off_snap + ((size_t)(2 * snap + 1) * G + g[b]) * C + cell(v)] = p[b][v];
+                    for (int v = 0; v < 4; ++v) {
+                        const int g = gene(b, v);
+                        if (g < G) {
+                            snap_pool[off_snap + ((size_t)(2 * snap) * G + g) * C + cell] = m[b][v];
+                            snap_pool[off_snap + ((size_t)(2 * snap + 1) * G + g) * C + cell] = p[b][v];
+                        }
                     }
                 }
             }
         }
     }
 
+    if (live) {
 #pragma unroll
-    for (int b = 0; b < NB; ++b) {
-        if (!gl[b]) continue;
+        for (int b = 0; b < NB; ++b) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            if (live(v)) {
-                state_pool[off_state + (size_t)g[b] * C + cell(v)] = m[b][v];
-                state_pool[off_state + (size_t)(G + g[b]) * C + cell(v)] = p[b][v];
-                prom_pool[off_prom + (size_t)g[b] * C + cell(v)] = (uint8_t)((on >> (4 * b + v)) & 1u);
+            for (int v = 0; v < 4; ++v) {
+                const int g = gene(b, v);
+                if (g < G) {
+                    state_pool[off_state + (size_t)g * C + cell] = m[b][v];
+                    state_pool[off_state + (size_t)(G + g) * C + cell] = p[b][v];
+                    prom_pool[off_prom + (size_t)g * C + cell] = (uint8_t)((on >> (4 * b + v)) & 1u);
+                }
             }
         }
     }
