@@ -58,6 +58,13 @@ COST_FP32 = 1.0       # v_add/mul/fma/max_f32
 COST_INT = 1.66       # 32-bit integer and logic (xor, add, shifts, compares)
 COST_MUL32 = 1.54     # one 32x32-bit product (v_mad_u64_u32, v_mul_hi_u32)
 COST_TRANS = 3.63     # v_log/exp/sqrt/rcp/sin/cos_f32
+# FP64 and other classes, from the same microbenchmark (the tau-leap's executed mix):
+COST_F64_FMA = 1.74   # v_fma_f64
+COST_F64_ADD = 1.60   # v_add_f64
+COST_F64_MUL = 1.65   # v_mul_f64
+COST_F64_TRANS = 6.08  # v_rcp/sqrt/rsq_f64
+COST_CVT = 1.53       # one f64 <-> i32 conversion
+COST_OTHER = 1.56     # any other 32-bit VALU instruction (moves, selects, logic, compares; v_perm_b32 measured)
 
 
 def log(msg: str) -> None:
@@ -403,6 +410,9 @@ def run_headline(a, sc) -> dict:
     g20 = run_g20(a, sc)
     log("serving workload")
     srv = run_serve(a, sc)
+    log("tau-leap workload")
+    tl_args = argparse.Namespace(**{**vars(a), "steps": min(a.steps, 3), "warmup": 1, "no_cpu": True})
+    tl = run_tauleap(tl_args, sc)
     out = {
         "metric": "cell*gene*steps/s, BF16-MFMA vs FP32 CLE at <=1% moment err; p99 latency",
         "value": bf["cgs_per_s"],
@@ -432,6 +442,7 @@ def run_headline(a, sc) -> dict:
         "precision": g,
         "g20": {k: g20[k] for k in ("value", "ms_per_step", "config", "roofline")},
         "serving": srv,
+        "tauleap": {k: tl.get(k) for k in ("workload", "cgs_per_s", "kernel_ms", "roofline")},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s (FP32-FMA-equivalent VALU issue)", "frac": achieved / FP32_PEAK_TFLOPS,
                      "traffic": traffic, "kernel": "cle_dense_kernel (BF16 MFMA)", "kernel_ms": bf["kernel_ms"],
@@ -445,6 +456,35 @@ def run_headline(a, sc) -> dict:
         out["cpu_baseline"] = primary
         out["cpu_baselines"] = {"host": host_cpu(), "threads_all": _threads_all(), "runs": entries}
     return out
+
+
+def tauleap_census():
+    """Executed VALU work per cell*gene*step of the tau-leap stepper on the G=20
+    workload, priced in FP32-FMA issue units.  Its samplers (inversion and
+    rejection, tauleap_binomial.jl) do data-dependent work, so there is no fixed
+    operation count: the census is the kernel's own instruction mix, counted by
+    rocprofv3 (profiles/tauleap_stepper_g20_mix.json, valid while the kernel
+    sources match its digest).  It includes the compiler's moves and address
+    arithmetic, so its fraction of peak is an issue-efficiency figure, an upper
+    bound on the algorithmic one."""
+    path = os.path.join(ROOT, "profiles", "tauleap_stepper_g20_mix.json")
+    if not os.path.exists(path):
+        return None
+    rec = json.load(open(path))
+    if rec.get("source_digest") != source_digest(rec["sources"]):
+        log("tauleap_stepper_g20_mix.json: measured on other kernel sources; census reported as null")
+        return None
+    m = rec["lane_instructions_per_cgs"]
+    classes = {"fma_f64": (m["SQ_INSTS_VALU_FMA_F64"], COST_F64_FMA), "add_f64": (m["SQ_INSTS_VALU_ADD_F64"], COST_F64_ADD),
+               "mul_f64": (m["SQ_INSTS_VALU_MUL_F64"], COST_F64_MUL),
+               "trans_f64": (m["SQ_INSTS_VALU_TRANS_F64"], COST_F64_TRANS),
+               "int64": (m["SQ_INSTS_VALU_INT64"], COST_MUL32), "int32": (m["SQ_INSTS_VALU_INT32"], COST_INT),
+               "cvt": (m["SQ_INSTS_VALU_CVT"], COST_CVT)}
+    other = m["SQ_INSTS_VALU"] - sum(n for n, _ in classes.values())
+    classes["other"] = (other, COST_OTHER)
+    fma_eq = sum(n * c for n, c in classes.values())
+    return {"lane_instructions": {k: round(n, 3) for k, (n, _) in classes.items()},
+            "valu_instructions": m["SQ_INSTS_VALU"], "fma_equiv_ops": fma_eq, "fp32_equiv_flops": 2.0 * fma_eq}
 
 
 def run_tauleap(a, sc) -> dict:
@@ -461,9 +501,17 @@ def run_tauleap(a, sc) -> dict:
     cgs = float(net.G) * cells * steps
     el, kms = timed_runs(plan, a.steps, a.warmup)
     y = plan.download(0).y
+    k_ms = float(np.mean(kms))
     out = {"workload": f"binomial tau-leap, G=20 telegraph network, {cells} cells, {steps} steps dt={dt}",
-           "cgs_per_s": cgs * a.steps / el, "kernel_ms": float(np.mean(kms)), "dtype": "f64 propensities, i32 counts",
+           "cgs_per_s": cgs * a.steps / el, "kernel_ms": k_ms, "dtype": "f64 propensities, i32 counts",
            "mean_mrna": float(y[:, :net.G].mean()), "mean_protein": float(y[:, net.G:].mean())}
+    census = tauleap_census() if cells == wl["cells"] else None
+    if census:
+        achieved = census["fp32_equiv_flops"] * cgs / (k_ms * 1e-3) / 1e12
+        out["roofline"] = {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                           "unit": "TFLOP/s (FP32-FMA-equivalent VALU issue, executed instruction mix)",
+                           "frac": achieved / FP32_PEAK_TFLOPS, "kernel": "tauleap_stepper_kernel",
+                           "census_per_cgs": census}
     if not a.no_cpu:
         from oracle import oracle as orc
         threads = max(1, min(16, os.cpu_count() or 1))

@@ -921,6 +921,7 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
     } while (0)
     PLAN_ALLOC(d_blobs, uint32_t, blobs.size());
     PLAN_ALLOC(d_tiles, Tile, all_tiles.size());
+    P->n_tiles_total = all_tiles.size();
     PLAN_ALLOC(d_state, float, state_elems);
     PLAN_ALLOC(d_snap, float, snap_elems);
     PLAN_ALLOC(d_prom, uint8_t, prom_elems);
@@ -1059,10 +1060,33 @@ int choose_segments(int n_tiles, long long resident, int64_t steps, int max_k) {
     return k < 1 ? 1 : (int)k;
 }
 
+// The dispatch a launch group makes, checked on the host before it is
+// enqueued (or recorded into a graph): a workgroup of 64..1024 threads, a
+// grid of 1..2^31-1 workgroups, dynamic plus static LDS within what the
+// device grants a workgroup, and a stream lane that exists.  The plan builder
+// only makes such groups; the check keeps a malformed dispatch packet from
+// ever reaching the device, also from a graph that replays it many times.
+static int check_dispatch(const scrna_plan* P, const LaunchGroup& lg, int nseg) {
+    const int threads = 64 * lg.NW;
+    const long long grid = (long long)lg.n_tiles * nseg;
+    const size_t lds = lg.lds + (lg.kind != 1 ? kStepperStaticLds : 0);
+    if (threads < 64 || threads > 1024 || grid < 1 || grid > INT32_MAX || lds > P->ctx->lds_max ||
+        lg.lane < 0 || lg.lane >= std::max(P->n_lanes, 1) || lg.tile0 < 0 ||
+        (size_t)lg.tile0 + (size_t)lg.n_tiles > P->n_tiles_total)
+        return set_error(SCRNA_ERR_DEVICE,
+                         "refusing a malformed dispatch: kind %d, %d threads, %lld workgroups, %zu B LDS (limit %zu), "
+                         "lane %d of %d", lg.kind, threads, grid, lds, P->ctx->lds_max, lg.lane, P->n_lanes);
+    return SCRNA_OK;
+}
+
 // Run steps [s_begin, s_end) of every network (clamped to each horizon).
 // A tile resumes from the state pool exactly when its first step is > 0.
 // `segmented` enables the segmented schedule of the FP32 and tau-leap stepper groups.
 int launch_steps(scrna_plan* P, hipStream_t st, int s_begin, int s_end, bool segmented) {
+    for (const LaunchGroup& lg : P->groups) {
+        int rc = check_dispatch(P, lg, segmented && lg.kind != 1 ? lg.nseg : 1);
+        if (rc) return rc;
+    }
     if (P->n_lanes > 1) {
         HIP_TRY(hipEventRecord(P->ev_fork, st));
         for (int i = 0; i + 1 < P->n_lanes; ++i) HIP_TRY(hipStreamWaitEvent(P->side[i], P->ev_fork, 0));

@@ -129,6 +129,7 @@ struct scrna_plan {
     // at ev_fork and join back through ev_join (also inside graph capture).
     static constexpr int kSideStreams = 3;
     int n_lanes = 1;
+    size_t n_tiles_total = 0;        // tiles of every launch group (d_tiles)
     hipStream_t side[kSideStreams] = {};
     hipEvent_t ev_fork = nullptr, ev_join[kSideStreams] = {};
     hipGraph_t graph = nullptr;
