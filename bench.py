@@ -126,7 +126,7 @@ def host_cpu() -> dict:
 
 
 def _threads_all() -> int:
-    # the GPU box gives one GPU a 16-core share whatever os.cpu_count() reports
+    # a one-GPU host share is 16 cores, whatever os.cpu_count() reports
     return max(1, min(16, os.cpu_count() or 1))
 
 

@@ -15,7 +15,7 @@ population stepper) against the oracle, with no statistical slack.
   half-integer, where the FP32 state may round the other way, or the cell
   holds a daughter of such a cell.  The marked share is reported and
   bounded.
-* The configs[3]-shaped case: 16 uneven networks from the batch sampler
+* The batched population workload's shape: 16 uneven networks from the batch sampler
   (G 5-64, telegraph, AND/OR), growth rate 0.05, division check every 10
   steps, binomial partitioning, uniform replacement, autotuned tiles, the
   whole run captured in one hipGraph and replayed.
@@ -123,7 +123,7 @@ def test_binomial_population_many_divisions(ctx, replace):
 
 
 def test_batch_workload_population_graph_autotuned():
-    """configs[3] shape at test size: 16 networks of the batch sampler, growth
+    """The 4096-network population batch at test size: 16 networks of the batch sampler, growth
     0.05, check every 10 steps, binomial partition, uniform replacement; tiles
     from the phase autotuner; the whole run in one hipGraph, replayed twice
     with eager work between, each replay equal to the eager plan and to the
