@@ -144,6 +144,13 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
     const bool live = cl < tile.ncell;
     const int cell = tile.cell0 + (live ? cl : 0);
     auto gene = [&](int b, int v) { return 16 * (w * NB + b) + 4 * qd + v; };
+    // The step-invariant first Philox round of the lane's normal blocks (cell, s, g>>1).
+    PhiloxFixed pfx[NB][2];
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+            pfx[b][h] = philox_fixed((uint32_t)cell, (uint32_t)(gene(b, 2 * h) >> 1), c3n, k1);
 
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
@@ -291,7 +298,7 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
         float nr2[NB][4], ncs[NB][4], nsn[NB][4];
         uint32_t nwa[NB][4], nwb[NB][4];
         auto draw = [&](const int b, const int h) {
-            const U4 u = philox4x32_10_uk((uint32_t)cell, (uint32_t)s, (uint32_t)(gene(b, 2 * h) >> 1), c3n, k0, k1);
+            const U4 u = philox4x32_10_fixed(pfx[b][h], (uint32_t)s, k0, k1);
             normal_pair_r2(u.x, u.y, neg2ln2dt, nr2[b][2 * h], ncs[b][2 * h], nsn[b][2 * h]);
             normal_pair_r2(u.z, u.w, neg2ln2dt, nr2[b][2 * h + 1], ncs[b][2 * h + 1], nsn[b][2 * h + 1]);
             nwa[b][2 * h] = u.x;

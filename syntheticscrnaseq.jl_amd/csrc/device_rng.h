@@ -72,6 +72,45 @@ __device__ __forceinline__ U4 philox4x32_10_uk(uint32_t c0, uint32_t c1, uint32_
     return philox4x32_10(c0, c1, c2, c3, k0, k1);
 }
 
+// A counter block whose c0, c2 and c3 stay fixed while c1 (the step) varies:
+// the first round's products M0*c0 and M1*c2 and the xor of hi(M0*c0) with c3
+// and the round-0 key k1 do not depend on c1, so they are computed once and
+// each block then starts from round 2 with one xor.  The result is the same
+// Philox4x32-10 block as philox4x32_10(c0, c1, c2, c3, k0, k1).
+struct PhiloxFixed {
+    uint32_t hi1, lo1;   // M1 * c2
+    uint32_t n2, lo0;    // hi(M0 * c0) ^ c3 ^ k1, lo(M0 * c0)
+};
+__device__ __forceinline__ PhiloxFixed philox_fixed(uint32_t c0, uint32_t c2, uint32_t c3, uint32_t k1) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c0;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
+    return PhiloxFixed{(uint32_t)(p1 >> 32), (uint32_t)p1, (uint32_t)(p0 >> 32) ^ c3 ^ k1, (uint32_t)p0};
+}
+// Block (c0, c1, c2, c3) from its fixed part, key wave-uniform (kept opaque
+// as in philox4x32_10_uk).
+__device__ __forceinline__ U4 philox4x32_10_fixed(const PhiloxFixed& f, uint32_t c1, uint32_t k0, uint32_t k1) {
+    asm volatile("" : "+s"(k0), "+s"(k1));
+    uint32_t c0 = f.hi1 ^ (c1 ^ k0), c2 = f.n2;
+    c1 = f.lo1;
+    uint32_t c3 = f.lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+#pragma unroll
+    for (int r = 1; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
+        const uint32_t n0 = r >= 2 ? xor3((uint32_t)(p1 >> 32), c1, k0) : (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = r >= 3 ? xor3((uint32_t)(p0 >> 32), c3, k1) : (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return U4{c0, c1, c2, c3};
+}
+
 // 1 + (w >> 9) * 2^-23 as a float in [1, 2): the top 23 bits of w placed
 // under the exponent of 1.0 by one v_alignbit_b32 ((0x7f:w) >> 9).
 __device__ __forceinline__ float one_plus_u23(uint32_t w) {
