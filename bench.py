@@ -562,6 +562,9 @@ def run_batch(a, sc) -> dict:
 
     n = a.networks
     nets, cells, pop = samplers.batch_networks(n)
+    if a.div_interval:
+        import dataclasses
+        pop = dataclasses.replace(pop, div_check_interval=a.div_interval)
     kin = samplers.WORKLOAD_KINETICS
     T, dt = a.T or 100.0, 0.01
     # Tile shapes (genes per lane per G bucket) from the autotuner, timed on this device.
@@ -575,7 +578,10 @@ def run_batch(a, sc) -> dict:
                                 div_check_interval=pop.div_check_interval, partition=pop.partition,
                                 replace=pop.replace) for c in cells]
     t0 = time.perf_counter()
-    plan = sc.Plan(nets, sc.CLE(dt), kin, population=pops, T=T, readout="mrna", seed=99)
+    if a.no_population:   # the same networks and cells as independent cells, one horizon launch
+        plan = sc.Plan(nets, sc.CLE(dt), kin, cell_num=list(cells), T=T, readout="mrna", seed=99)
+    else:
+        plan = sc.Plan(nets, sc.CLE(dt), kin, population=pops, T=T, readout="mrna", seed=99)
     t_build = time.perf_counter() - t0
     t0 = time.perf_counter()
     if not a.no_graph:
@@ -585,8 +591,9 @@ def run_batch(a, sc) -> dict:
     steps = sc.n_steps(T, dt)
     cgs = float(sum(net.G * c for net, c in zip(nets, cells))) * steps
     sample = plan.download(0)
-    return {"workload": f"{n} networks, G~U{{5..64}}, cells logU[256,8192], population (growth 0.05, V_div 2, "
-                        f"binomial partition, Moran), telegraph, T={T}, dt={dt}, "
+    popdesc = ("no population" if a.no_population else
+               f"population (growth 0.05, V_div 2, binomial partition, Moran, check every {pop.div_check_interval})")
+    return {"workload": f"{n} networks, G~U{{5..64}}, cells logU[256,8192], {popdesc}, telegraph, T={T}, dt={dt}, "
                         f"{'direct launches' if a.no_graph else 'one hipGraph'}",
             "networks": n, "total_cells": int(sum(cells)), "mean_G": float(np.mean([x.G for x in nets])),
             "cell_gene_steps": cgs, "cgs_per_s": cgs * a.steps / el, "s_per_run": el / a.steps,
@@ -678,6 +685,9 @@ def main() -> int:
     ap.add_argument("--networks", type=int, default=4096, help="batch: number of networks")
     ap.add_argument("--T", type=float, default=None, help="batch: horizon")
     ap.add_argument("--no-graph", action="store_true", help="batch: launch the plan's kernels directly, no hipGraph")
+    ap.add_argument("--div-interval", type=int, default=0, help="batch: division check interval (default 10)")
+    ap.add_argument("--no-population", action="store_true",
+                    help="batch: the same cells without population dynamics, one horizon launch")
     ap.add_argument("--qps", type=float, default=4.0, help="serve: arrival rate")
     ap.add_argument("--requests", type=int, default=60, help="serve: number of requests")
     ap.add_argument("--bytes", type=int, default=1 << 30, help="copy: bytes per copy")
