@@ -100,3 +100,33 @@ def test_fp32_precision_policy(ctx):
                                        n_and=(1, 2), n_or=(1, 2))
     b = sc.simulate(sparse, sc.CLE(0.01), KIN, precision="fp32", **kw)
     np.testing.assert_array_equal(b, sc.simulate(sparse, sc.CLE(0.01), KIN, precision="fp32_valu", **kw))
+
+
+@pytest.mark.parametrize("precision,tol", [("bf16x3", 2e-4), ("bf16", 2e-2), ("fp32_mfma", 1e-5)])
+def test_dense_snapshots_and_moments(ctx, precision, tol):
+    """Snapshots of the dense stepper (each lane writes its four genes of one
+    cell): against the oracle's snapshots on injected noise, with G = 40 (24
+    padding genes in the last block), 37 cells (a ragged tile), the last
+    snapshot equal to the readout and the FP64 moments equal to sums of the
+    snapshots."""
+    rng = np.random.default_rng(11)
+    G, cells, dt, T, every = 40, 37, 0.005, 0.6, 30
+    net = sc.samplers.mixed_network(G, rng, n_tf=(6, 8), targets_per_tf=(5, 12), n_and=(2, 3), n_or=(2, 3),
+                                    a_range=(0.5, 3.0))
+    steps = sc.n_steps(T, dt)
+    xi = rng.standard_normal((steps, G, 2, cells)).astype(np.float32)
+    print(f"dense snapshots {precision}", flush=True)
+    res = sc.simulate(net, sc.CLE(dt), KIN, cell_num=cells, T=T, readout="both", injected_noise=xi, seed=3,
+                      precision=precision, snapshot_every=every, full=True)
+    ref = orc.cle(net, KIN, dt=dt, n_steps=steps, cell_num=cells, injected=xi, seed=3, snapshot_every=every)
+    n_snap = steps // every
+    assert res.snapshots.shape == (n_snap, 2, G, cells)
+    for k in range(n_snap):
+        assert relerr(res.snapshots[k, 0].astype(np.float64), ref["snaps_m"][k]) <= tol
+        assert relerr(res.snapshots[k, 1].astype(np.float64), ref["snaps_p"][k]) <= tol
+    np.testing.assert_array_equal(res.snapshots[-1, 0].T.astype(np.float64), res.y[:, :G])
+    np.testing.assert_array_equal(res.snapshots[-1, 1].T.astype(np.float64), res.y[:, G:])
+    for k in range(n_snap):
+        want = orc.moments(res.snapshots[k, 0].astype(np.float64), res.snapshots[k, 1].astype(np.float64))
+        np.testing.assert_allclose(res.moments[k], want, rtol=1e-12, atol=1e-9)
+
