@@ -194,6 +194,33 @@ function _simulate(net, alg, kin, pop, cap; cell_num, T, readout, seed, init=:st
     return Y, Matrix{Int}(counts)
 end
 
+# No counterpart in the reference: the CLE of one network without population
+# dynamics, with every device buffer taken from the library's fixed per-device
+# array (scrna_simulate_cle_noalloc), for callers that must not allocate device
+# memory.  Larger runs than the array holds throw ArgumentError; calls on one
+# device are serialised.
+function simulate_noalloc(net::GeneNetwork, alg::CLE, kin::KineticParams; cell_num::Int=1000, T::Float64=50.0,
+                          readout::Symbol=:protein, seed::UInt64=rand(Random.RandomDevice(), UInt64),
+                          init::Symbol=:steady, precision::Symbol=:fp32, capture=nothing)
+    keep = Any[]
+    cn = Ref(_cnet(net, keep))
+    ck = Ref(_ckin(kin))
+    cr = Ref(_crun(alg, cell_num, T, readout, seed, 0; init, precision))
+    cc = Ref(_ccap(capture))
+    ncol = readout === :both ? 2 * net.G : net.G
+    Y = Matrix{Float64}(undef, cell_num, ncol)
+    counts = capture === nothing ? Matrix{Int32}(undef, 0, 0) : Matrix{Int32}(undef, cell_num, net.G)
+    out = Ref(COutput(pointer(Y), Int64(cell_num), capture === nothing ? Ptr{Int32}(0) : pointer(counts),
+                      Ptr{Float32}(0), Ptr{Float64}(0), Ptr{Float64}(0)))
+    GC.@preserve keep Y counts begin
+        rc = ccall((:scrna_simulate_cle_noalloc, LIB), Cint,
+                   (Ptr{Cvoid}, Ref{CNetwork}, Ref{CKinetics}, Ref{CRun}, Ref{CCapture}, Ref{COutput}),
+                   context(), cn, ck, cr, cc, out)
+    end
+    _check(rc, "simulate_noalloc")
+    return capture === nothing ? Y : (Y, Matrix{Int}(counts))
+end
+
 # ext/SyntheticscRNAseqCUDAExt.jl:132-213 and 317-399 (networks may differ in G here)
 function SyntheticscRNAseq.simulate_gpu_batch(networks::Vector{GeneNetwork}, alg::GpuAlg, kin::KineticParams;
                                               cell_num::Int=1000, T::Float64=50.0, readout::Symbol=:protein,

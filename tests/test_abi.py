@@ -80,3 +80,16 @@ def test_struct_layout_matches_c():
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(_lib.ScrnaError):
         _lib.load(str(tmp_path / "nope.so"))
+
+
+def test_julia_shim_binds_only_exported_symbols():
+    """Every ccall of the Julia shim names a symbol the library exports and the
+    header declares (the shim cannot run here: no Julia)."""
+    shim = os.path.join(ROOT, "syntheticscrnaseq.jl_amd", "julia", "SyntheticscRNAseqHIP.jl")
+    called = set(re.findall(r"ccall\(\(:(\w+), LIB\)", open(shim).read()))
+    assert {"scrna_simulate_cle", "scrna_simulate_cle_batch", "scrna_simulate_cle_noalloc"} <= called
+    lib = C.CDLL(_lib.LIB_PATH)
+    declared = set(declared_functions())
+    for name in sorted(called):
+        assert name in declared, name
+        assert hasattr(lib, name), name
