@@ -49,23 +49,48 @@ int population_prepare(scrna_plan* P, size_t state_elems, size_t prom_elems, siz
     return SCRNA_OK;
 }
 
+// Each stream lane owns whole networks (their tiles contiguous in d_tiles)
+// and runs their phases and divisions on its own stream; a network's
+// division only ever reads and writes that network's cells, so the lanes
+// need to meet only at the end.
 int population_launch(scrna_plan* P, hipStream_t st) {
     const int I = P->pop_interval;
     const int64_t n_max = P->pop_max_steps;
     int n_tiles = 0;
     for (auto& lg : P->groups) n_tiles += lg.n_tiles;
     HIP_TRY_P(launch_pop_init(n_tiles, P->d_blobs, P->d_tiles, P->d_vol, P->d_vinit, st));
+    const int L = std::max(P->n_lanes, 1);
+    if (L == 1) {
+        P->lane_t0[0] = 0;
+        P->lane_nt[0] = n_tiles;
+    }
+    auto lane_stream = [&](int l) { return l == 0 ? st : P->side[l - 1]; };
+    if (L > 1) {
+        HIP_TRY_P(hipEventRecord(P->ev_fork, st));
+        for (int i = 0; i + 1 < L; ++i) HIP_TRY_P(hipStreamWaitEvent(P->side[i], P->ev_fork, 0));
+    }
     for (int64_t s0 = 0; s0 < n_max || s0 == 0; s0 += I) {
         const int64_t s1 = std::min<int64_t>(s0 + I, n_max);
-        int rc = launch_steps(P, st, (int)s0, (int)s1, false);
-        if (rc) return rc;
-        if (s1 > s0 && s1 % I == 0) {   // cle.jl:240 (step % interval == 0, 1-based step = s1)
-            HIP_TRY_P(launch_pop_divide(n_tiles, P->d_blobs, P->d_tiles, P->d_state, P->d_prom, P->d_vol,
-                                        P->d_dstate, P->d_dprom, P->d_dvol, P->d_claim, (int)(s1 - 1), st));
-            HIP_TRY_P(launch_pop_place(n_tiles, P->d_blobs, P->d_tiles, P->d_state, P->d_prom, P->d_vol,
-                                       P->d_dstate, P->d_dprom, P->d_dvol, P->d_claim, st));
+        for (int l = 0; l < L; ++l) {
+            hipStream_t gs = lane_stream(l);
+            for (const LaunchGroup& lg : P->groups) {
+                if (lg.lane != l) continue;
+                int rc = launch_group_steps(P, lg, gs, (int)s0, (int)s1, false);
+                if (rc) return rc;
+            }
+            if (s1 > s0 && s1 % I == 0 && P->lane_nt[l] > 0) {   // cle.jl:240 (step % interval == 0, 1-based step = s1)
+                const Tile* tiles = P->d_tiles + P->lane_t0[l];
+                HIP_TRY_P(launch_pop_divide(P->lane_nt[l], P->d_blobs, tiles, P->d_state, P->d_prom, P->d_vol,
+                                            P->d_dstate, P->d_dprom, P->d_dvol, P->d_claim, (int)(s1 - 1), gs));
+                HIP_TRY_P(launch_pop_place(P->lane_nt[l], P->d_blobs, tiles, P->d_state, P->d_prom, P->d_vol,
+                                           P->d_dstate, P->d_dprom, P->d_dvol, P->d_claim, gs));
+            }
         }
         if (n_max == 0) break;
+    }
+    for (int i = 0; i + 1 < L; ++i) {
+        HIP_TRY_P(hipEventRecord(P->ev_join[i], P->side[i]));
+        HIP_TRY_P(hipStreamWaitEvent(st, P->ev_join[i], 0));
     }
     return SCRNA_OK;
 }

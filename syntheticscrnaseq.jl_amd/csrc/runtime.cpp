@@ -812,11 +812,35 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
             hb[H_DENSE_HI] = (uint32_t)((uint64_t)cn.off_w >> 32);
         }
     }
+    // Population plans fix each launch group's stream lane here (largest work
+    // first, each to the least loaded lane) and store a lane's tiles together,
+    // so that the lane can run the division of its own networks and go on to
+    // its next phase without waiting for the other lanes (population_launch).
+    std::vector<std::pair<int, GKey>> key_order;
+    for (auto& kv : groups) key_order.push_back({0, kv.first});
+    if (P->has_population && !arena) {
+        auto gcost = [&](const GKey& k) { return (double)groups[k].size() * k.NW * std::max(1, k.a); };
+        std::vector<size_t> idx(key_order.size());
+        for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+        std::stable_sort(idx.begin(), idx.end(),
+                         [&](size_t x, size_t y) { return gcost(key_order[x].second) > gcost(key_order[y].second); });
+        const int L = (int)std::min<size_t>(1 + scrna_plan::kSideStreams, key_order.size());
+        std::vector<double> load((size_t)std::max(1, L), 0.0);
+        for (size_t i : idx) {
+            const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            key_order[i].first = l;
+            load[l] += gcost(key_order[i].second);
+        }
+        std::stable_sort(key_order.begin(), key_order.end(),
+                         [](const std::pair<int, GKey>& x, const std::pair<int, GKey>& y) { return x.first < y.first; });
+    }
     std::vector<Tile> all_tiles;
-    for (auto& kv : groups) {
+    for (auto& lk : key_order) {
+        const auto& kv = *groups.find(lk.second);
         LaunchGroup lg;
         lg.kind = kv.first.kind;
         lg.NW = kv.first.NW;
+        lg.lane = lk.first;
         lg.tile0 = (int)all_tiles.size();
         lg.n_tiles = (int)kv.second.size();
         const int S = std::max(1, group_SG[kv.first].first);
@@ -1013,12 +1037,22 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
             return (double)g.n_tiles * (double)g.NW * (double)std::max(1, g.CH);
         };
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
-        P->n_lanes = (int)std::min<size_t>(1 + scrna_plan::kSideStreams, P->groups.size());
-        std::vector<double> load((size_t)std::max(1, P->n_lanes), 0.0);
-        for (int i : order) {
-            const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-            P->groups[i].lane = l;
-            load[l] += cost(i);
+        if (P->has_population) {
+            // lanes were fixed per launch group when the groups were formed
+            P->n_lanes = 1;
+            for (const auto& g : P->groups) P->n_lanes = std::max(P->n_lanes, g.lane + 1);
+            for (const auto& g : P->groups) {
+                if (P->lane_nt[g.lane] == 0) P->lane_t0[g.lane] = g.tile0;
+                P->lane_nt[g.lane] += g.n_tiles;
+            }
+        } else {
+            P->n_lanes = (int)std::min<size_t>(1 + scrna_plan::kSideStreams, P->groups.size());
+            std::vector<double> load((size_t)std::max(1, P->n_lanes), 0.0);
+            for (int i : order) {
+                const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+                P->groups[i].lane = l;
+                load[l] += cost(i);
+            }
         }
         bool ok = true;
         for (int i = 0; ok && i + 1 < P->n_lanes; ++i)
@@ -1082,6 +1116,33 @@ static int check_dispatch(const scrna_plan* P, const LaunchGroup& lg, int nseg) 
 // Run steps [s_begin, s_end) of every network (clamped to each horizon).
 // A tile resumes from the state pool exactly when its first step is > 0.
 // `segmented` enables the segmented schedule of the FP32 and tau-leap stepper groups.
+int launch_group_steps(scrna_plan* P, const LaunchGroup& lg, hipStream_t gs, int s_begin, int s_end,
+                       bool segmented) {
+    int rc = check_dispatch(P, lg, segmented && lg.kind != 1 ? lg.nseg : 1);
+    if (rc) return rc;
+    if (lg.kind == 1) {
+        HIP_TRY(launch_dense(lg.mode, lg.tele, lg.inject, lg.clamp, lg.NW, lg.n_tiles, P->d_blobs, P->d_tiles + lg.tile0,
+                             P->d_wpool, P->d_state, P->d_snap, P->d_prom, P->d_inj, s_begin, s_end, lg.lds, gs));
+        return SCRNA_OK;
+    }
+    int* sched = nullptr;
+    int nseg = 1;
+    if (segmented && lg.nseg > 1) {
+        sched = P->d_sched + lg.sched_off;
+        nseg = lg.nseg;
+        HIP_TRY(launch_sched_reset(sched, (int)sched_words(lg.n_tiles), gs));
+    }
+    if (lg.kind == 0)
+        HIP_TRY(launch_stepper(lg.CH, lg.NW, lg.n_tiles, P->d_blobs, P->d_tiles + lg.tile0, P->d_state, P->d_snap,
+                               P->d_prom, P->d_vol, P->d_inj, s_begin, s_end, lg.nbuf, lg.lds, sched, nseg,
+                               P->wait_ticks, lg.tele, lg.clamp, gs));
+    else
+        HIP_TRY(launch_tauleap(lg.CH, lg.NW, lg.n_tiles, P->d_blobs, P->d_tiles + lg.tile0, P->d_state, P->d_snap,
+                               P->d_prom, P->d_vol, s_begin, s_end, lg.nbuf, lg.lds, sched, nseg, P->wait_ticks,
+                               gs));
+    return SCRNA_OK;
+}
+
 int launch_steps(scrna_plan* P, hipStream_t st, int s_begin, int s_end, bool segmented) {
     for (const LaunchGroup& lg : P->groups) {
         int rc = check_dispatch(P, lg, segmented && lg.kind != 1 ? lg.nseg : 1);
@@ -1093,26 +1154,8 @@ int launch_steps(scrna_plan* P, hipStream_t st, int s_begin, int s_end, bool seg
     }
     for (const LaunchGroup& lg : P->groups) {
         hipStream_t gs = lg.lane == 0 ? st : P->side[lg.lane - 1];
-        if (lg.kind == 0 || lg.kind == 2) {
-            int* sched = nullptr;
-            int nseg = 1;
-            if (segmented && lg.nseg > 1) {
-                sched = P->d_sched + lg.sched_off;
-                nseg = lg.nseg;
-                HIP_TRY(launch_sched_reset(sched, (int)sched_words(lg.n_tiles), gs));
-            }
-            if (lg.kind == 0)
-                HIP_TRY(launch_stepper(lg.CH, lg.NW, lg.n_tiles, P->d_blobs, P->d_tiles + lg.tile0, P->d_state,
-                                       P->d_snap, P->d_prom, P->d_vol, P->d_inj, s_begin, s_end, lg.nbuf, lg.lds,
-                                       sched, nseg, P->wait_ticks, lg.tele, lg.clamp, gs));
-            else
-                HIP_TRY(launch_tauleap(lg.CH, lg.NW, lg.n_tiles, P->d_blobs, P->d_tiles + lg.tile0, P->d_state,
-                                       P->d_snap, P->d_prom, P->d_vol, s_begin, s_end, lg.nbuf, lg.lds, sched,
-                                       nseg, P->wait_ticks, gs));
-        } else
-            HIP_TRY(launch_dense(lg.mode, lg.tele, lg.inject, lg.clamp, lg.NW, lg.n_tiles, P->d_blobs, P->d_tiles + lg.tile0,
-                                 P->d_wpool, P->d_state, P->d_snap, P->d_prom, P->d_inj, s_begin, s_end, lg.lds,
-                                 gs));
+        int rc = launch_group_steps(P, lg, gs, s_begin, s_end, segmented);
+        if (rc) return rc;
     }
     for (int i = 0; i + 1 < P->n_lanes; ++i) {
         HIP_TRY(hipEventRecord(P->ev_join[i], P->side[i]));

@@ -130,6 +130,10 @@ struct scrna_plan {
     static constexpr int kSideStreams = 3;
     int n_lanes = 1;
     size_t n_tiles_total = 0;        // tiles of every launch group (d_tiles)
+    // Population plans: each lane owns whole networks, whose tiles are
+    // contiguous in d_tiles ([lane_t0, lane_t0 + lane_nt)), and advances its
+    // phases and divisions on its own, joining the others only at the end.
+    int lane_t0[1 + kSideStreams] = {}, lane_nt[1 + kSideStreams] = {};
     hipStream_t side[kSideStreams] = {};
     hipEvent_t ev_fork = nullptr, ev_join[kSideStreams] = {};
     hipGraph_t graph = nullptr;
@@ -164,6 +168,8 @@ int plan_download(scrna_plan* P, int32_t i, scrna_output* out);
 void plan_release(scrna_plan* P);
 int population_launch(scrna_plan* P, hipStream_t st);
 int launch_steps(scrna_plan* P, hipStream_t st, int s_begin, int s_end, bool segmented);
+// One launch group's stepper launch on stream gs (checked on the host first).
+int launch_group_steps(scrna_plan* P, const LaunchGroup& lg, hipStream_t gs, int s_begin, int s_end, bool segmented);
 int choose_segments(int n_tiles, long long resident, int64_t steps, int max_k);
 int plan_make_graph(scrna_plan* P);
 int unfused_prepare(scrna_plan* P, UnfusedNet& u);
