@@ -59,11 +59,7 @@ int population_launch(scrna_plan* P, hipStream_t st) {
     int n_tiles = 0;
     for (auto& lg : P->groups) n_tiles += lg.n_tiles;
     HIP_TRY_P(launch_pop_init(n_tiles, P->d_blobs, P->d_tiles, P->d_vol, P->d_vinit, st));
-    const int L = std::max(P->n_lanes, 1);
-    if (L == 1) {
-        P->lane_t0[0] = 0;
-        P->lane_nt[0] = n_tiles;
-    }
+    const int L = std::max(P->n_lanes, 1);   // lanes and their tile ranges are set by plan_build
     auto lane_stream = [&](int l) { return l == 0 ? st : P->side[l - 1]; };
     if (L > 1) {
         HIP_TRY_P(hipEventRecord(P->ev_fork, st));
