@@ -21,6 +21,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "blob.h"
 #include "device_binomial.h"
 #include "device_rng.h"
@@ -831,28 +833,45 @@ hipError_t set_stepper_lds_limit(size_t bytes) {
     return e;
 }
 
+// The job of a capture or moment workgroup is blockIdx.y, whose range is
+// at most 65535: plans with more jobs launch them in chunks.
+constexpr int kMaxGridY = 65535;
+
+template <typename T>
+static hipError_t launch_capture(int n_jobs, int64_t max_elems, const CaptureJob* jobs, const T* src,
+                                 int32_t* dst, hipStream_t stream) {
+    if (n_jobs <= 0 || max_elems <= 0) return hipSuccess;
+    const int64_t gx = (max_elems + 255) / 256;
+    if (gx > INT32_MAX) return hipErrorInvalidConfiguration;
+    for (int j0 = 0; j0 < n_jobs; j0 += kMaxGridY) {
+        const dim3 grid((unsigned)gx, (unsigned)std::min(kMaxGridY, n_jobs - j0)), block(256);
+        hipLaunchKernelGGL(capture_kernel<T>, grid, block, 0, stream, jobs + j0, src, dst);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 hipError_t launch_capture_f32(int n_jobs, int64_t max_elems, const CaptureJob* jobs, const float* src,
                               int32_t* dst, hipStream_t stream) {
-    if (n_jobs <= 0 || max_elems <= 0) return hipSuccess;
-    const dim3 grid((unsigned)((max_elems + 255) / 256), (unsigned)n_jobs), block(256);
-    hipLaunchKernelGGL(capture_kernel<float>, grid, block, 0, stream, jobs, src, dst);
-    return hipGetLastError();
+    return launch_capture<float>(n_jobs, max_elems, jobs, src, dst, stream);
 }
 
 hipError_t launch_capture_f64(int n_jobs, int64_t max_elems, const CaptureJob* jobs, const double* src,
                               int32_t* dst, hipStream_t stream) {
-    if (n_jobs <= 0 || max_elems <= 0) return hipSuccess;
-    const dim3 grid((unsigned)((max_elems + 255) / 256), (unsigned)n_jobs), block(256);
-    hipLaunchKernelGGL(capture_kernel<double>, grid, block, 0, stream, jobs, src, dst);
-    return hipGetLastError();
+    return launch_capture<double>(n_jobs, max_elems, jobs, src, dst, stream);
 }
 
 hipError_t launch_moments(int n_jobs, int max_G, const MomentJob* jobs, const float* pool, double* out,
                           hipStream_t stream) {
     if (n_jobs <= 0 || max_G <= 0) return hipSuccess;
-    const dim3 grid((unsigned)max_G, (unsigned)n_jobs), block(256);
-    hipLaunchKernelGGL(moments_kernel, grid, block, 0, stream, jobs, pool, out);
-    return hipGetLastError();
+    for (int j0 = 0; j0 < n_jobs; j0 += kMaxGridY) {
+        const dim3 grid((unsigned)max_G, (unsigned)std::min(kMaxGridY, n_jobs - j0)), block(256);
+        hipLaunchKernelGGL(moments_kernel, grid, block, 0, stream, jobs + j0, pool, out);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace scrna

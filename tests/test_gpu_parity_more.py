@@ -116,3 +116,24 @@ def test_large_network_runs_reference_structured_path(ctx):
         sc.simulate(tele, sc.CLE(dt), kin, cell_num=10, T=0.1)
     with pytest.raises(sc.UnsupportedError, match="reference-structured"):
         sc.simulate(net, sc.CLE(dt), kin, cell_num=10, T=0.1, precision="bf16")
+
+
+def test_more_networks_than_one_grid_row(ctx):
+    """66,000 one-gene networks in one plan: the capture and moment launches
+    index networks by blockIdx.y (at most 65,535) and run in chunks.  The
+    networks past the first chunk are checked against the oracle's capture of
+    their own state and against FP64 moments."""
+    n = 66000
+    nets = [sc.GeneNetwork(1, np.array([1.0 + (i % 7)]), np.zeros((1, 1))) for i in range(n)]
+    cap = sc.CaptureModel(efficiency=0.3, efficiency_std=0.2, readout="mrna")
+    print(f"{n} networks, capture and moments", flush=True)
+    kin = sc.KineticParams(k_t=5.0, K_d=10.0, n=2.0, mu_m=1.0, mu_p=0.2)
+    out = sc.simulate_gpu_batch(nets, sc.CLE(0.01), kin, cell_num=3, T=0.05, readout="mrna", seed=4, capture=cap,
+                                full=True)
+    for i in (0, 65534, 65535, 65536, n - 1):
+        r = out[i]
+        want, _ = orc.capture(r.y.T, 0.3, 0.2, 4, net_id=i)
+        np.testing.assert_array_equal(r.counts, want.T)
+        m = r.y[:, 0].astype(np.float32).astype(np.float64)
+        assert r.moments.shape[-1] == 5
+        np.testing.assert_allclose(r.moments[0, 0, 0], m.sum(), rtol=1e-12)
