@@ -72,14 +72,16 @@ __device__ inline int64_t binomial_sample(int64_t n, double p, UStream& st, cons
         const double r0 = r0_known >= 0.0 ? r0_known
                           : lq            ? exp((double)n * (flip ? lq[1] : lq[0]))
                                           : pow(q, (double)n);
+        // 32-bit counter: the loop ends long before 2^31 (k exceeding n redraws)
+        const int nk = n > (int64_t)INT32_MAX ? INT32_MAX : (int)n;
         for (;;) {
             double u = st.next(), r = r0;
-            int64_t k = 0;
+            int k = 0;
             bool ok = true;
             while (u > r) {
                 u -= r;
                 ++k;
-                if (k > n) { ok = false; break; }
+                if (k > nk) { ok = false; break; }
                 r *= (a / (double)k - s);
             }
             if (ok) { x = k; break; }
@@ -162,7 +164,7 @@ __device__ inline int64_t poisson_sample(double lam, UStream& st, double p0_know
         const double p0 = p0_known >= 0.0 ? p0_known : exp(-lam);
         for (;;) {
             double u = st.next(), p = p0;
-            int64_t k = 0;
+            int k = 0;
             while (u > p) {
                 u -= p;
                 ++k;
