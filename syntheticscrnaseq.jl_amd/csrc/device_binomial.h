@@ -58,20 +58,26 @@ __device__ __forceinline__ double stirling_fc(int64_t k) {
 // the inversion start q^n is exp(n * log q) instead of pow(q, n) (a fixed
 // decay probability drawn every step).  r0_known >= 0 is that start value
 // itself, exp(n * log q) computed ahead (a table over n).
-__device__ inline int64_t binomial_sample(int64_t n, double p, UStream& st, const double* lq = nullptr,
+// I: the count's integer type (int for the tau-leap's int32 counts, whose
+// conversion to double is one instruction; int64_t for capture and division).
+// (double)n + 1.0 equals (double)(n + 1) for every count below 2^53.
+template <typename I>
+__device__ inline int64_t binomial_sample(I n_in, double p, UStream& st, const double* lq = nullptr,
                                           double r0_known = -1.0) {
 #pragma clang fp contract(off)
-    if (n <= 0 || !(p > 0.0)) return 0;
+    const int64_t n = n_in;
+    if (n_in <= 0 || !(p > 0.0)) return 0;
     if (p >= 1.0) return n;
     const bool flip = p > 0.5;
     const double pp = flip ? 1.0 - p : p;
     const double q = 1.0 - pp;
+    const double nd = (double)n_in;
     int64_t x = 0;
-    if ((double)n * pp < 10.0) {
-        const double s = pp / q, a = (double)(n + 1) * s;
+    if (nd * pp < 10.0) {
+        const double s = pp / q, a = (nd + 1.0) * s;
         const double r0 = r0_known >= 0.0 ? r0_known
-                          : lq            ? exp((double)n * (flip ? lq[1] : lq[0]))
-                                          : pow(q, (double)n);
+                          : lq            ? exp(nd * (flip ? lq[1] : lq[0]))
+                                          : pow(q, nd);
         // 32-bit counter: the loop ends long before 2^31 (k exceeding n redraws)
         const int nk = n > (int64_t)INT32_MAX ? INT32_MAX : (int)n;
         for (;;) {

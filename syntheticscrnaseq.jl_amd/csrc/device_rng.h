@@ -146,9 +146,12 @@ __device__ __forceinline__ void normal_pair_r2(uint32_t a, uint32_t b, float neg
 }
 
 // 52-bit uniform in (0,1) from two u32 (capture and population draws).
+// Computed as a * 2^20 + (b >> 12) + 1/2 in doubles: every intermediate is an
+// integer or half-integer below 2^52, so each operation is exact and the value
+// is that of the 52-bit integer x = (a << 20) | (b >> 12), without a 64-bit
+// integer conversion.
 __device__ __forceinline__ double u52(uint32_t a, uint32_t b) {
-    const uint64_t x = ((uint64_t)a << 20) | (uint64_t)(b >> 12);
-    return ((double)x + 0.5) * 0x1p-52;
+    return fma((double)a, 0x1p20, (double)(b >> 12) + 0.5) * 0x1p-52;
 }
 
 }  // namespace scrna
