@@ -679,7 +679,7 @@ def main() -> int:
     ap.add_argument("--cells", type=int, default=None, help="override cell count (testing only)")
     ap.add_argument("--tile", type=int, default=0, help="genes per lane for the FP32 stepper (2, 4, 8, 16)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--subset", type=int, default=256, help="g256: cells replayed by the FP64 oracle")
+    ap.add_argument("--subset", type=int, default=1024, help="g256: cells replayed by the FP64 oracle")
     ap.add_argument("--precisions", default="fp32_unfused,fp32_valu,fp32_mfma,bf16,bf16x3",
                     help="g256: comma-separated forms")
     ap.add_argument("--networks", type=int, default=4096, help="batch: number of networks")
