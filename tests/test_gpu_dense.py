@@ -22,7 +22,7 @@ def relerr(a, b):
 
 
 @pytest.mark.parametrize("precision,tol", [("bf16x3", 2e-4), ("bf16", 2e-2), ("fp32_mfma", 1e-5)])
-@pytest.mark.parametrize("G,cells", [(40, 50), (96, 33)])
+@pytest.mark.parametrize("G,cells", [(40, 50), (96, 33), (130, 21), (200, 17)])
 def test_dense_injected_parity(ctx, precision, tol, G, cells):
     rng = np.random.default_rng(G)
     net = sc.samplers.mixed_network(G, rng, n_tf=(6, 8), targets_per_tf=(5, 12), n_and=(2, 3), n_or=(2, 3),
