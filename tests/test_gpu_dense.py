@@ -130,3 +130,25 @@ def test_dense_snapshots_and_moments(ctx, precision, tol):
         want = orc.moments(res.snapshots[k, 0].astype(np.float64), res.snapshots[k, 1].astype(np.float64))
         np.testing.assert_allclose(res.moments[k], want, rtol=1e-12, atol=1e-9)
 
+
+
+@pytest.mark.parametrize("precision,tol", [("bf16x3", 2e-4), ("fp32_mfma", 1e-5)])
+def test_dense_negative_initial_counts(ctx, precision, tol):
+    """Negative basal rates start some genes below zero (ceil(beta / mu_m) < 0):
+    the dense kernel's clamped variant keeps max(m, 0), max(p, 0) and the
+    protein variance's clamp on the first step, as cle.jl:109-120 does."""
+    rng = np.random.default_rng(29)
+    base = sc.samplers.mixed_network(40, rng, n_tf=(6, 8), targets_per_tf=(5, 12), n_and=(2, 3), n_or=(2, 3),
+                                     a_range=(0.5, 3.0))
+    basals = np.array(base.basals, dtype=np.float64)
+    basals[::5] = -np.abs(basals[::5]) - 0.5
+    net = sc.GeneNetwork(base.G, basals, np.array(base.interactions), list(base.cooperative), list(base.redundant))
+    cells, dt, T = 40, 0.005, 0.5
+    steps = sc.n_steps(T, dt)
+    xi = rng.standard_normal((steps, net.G, 2, cells)).astype(np.float32)
+    print(f"dense negative init {precision}", flush=True)
+    y = sc.simulate(net, sc.CLE(dt), KIN, cell_num=cells, T=T, readout="both", injected_noise=xi, seed=3,
+                    precision=precision)
+    ref = orc.cle(net, KIN, dt=dt, n_steps=steps, cell_num=cells, injected=xi, seed=3)
+    assert np.all(y >= 0)
+    assert relerr(y, np.concatenate([ref["m"].T, ref["p"].T], axis=1)) <= tol
