@@ -152,3 +152,18 @@ def test_dense_negative_initial_counts(ctx, precision, tol):
     ref = orc.cle(net, KIN, dt=dt, n_steps=steps, cell_num=cells, injected=xi, seed=3)
     assert np.all(y >= 0)
     assert relerr(y, np.concatenate([ref["m"].T, ref["p"].T], axis=1)) <= tol
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32_mfma", 2e-4), ("bf16x3", 5e-4)])
+def test_dense_telegraph_philox_replay(ctx, precision, tol):
+    """Free Philox noise with telegraph promoters (the switching bits come from
+    the normal words drawn beside the MFMAs): the oracle replays the same
+    stream, so switching decisions agree exactly and states to FP32 rounding."""
+    rng = np.random.default_rng(31)
+    net = sc.samplers.mixed_network(48, rng, n_tf=(4, 5), targets_per_tf=(3, 6), n_and=(1, 2), n_or=(1, 2),
+                                    telegraph=True)
+    cells, dt, T = 70, 0.01, 1.5
+    print(f"dense telegraph replay {precision}", flush=True)
+    y = sc.simulate(net, sc.CLE(dt), KIN, cell_num=cells, T=T, readout="both", seed=12, precision=precision)
+    ref = orc.cle(net, KIN, dt=dt, T=T, cell_num=cells, seed=12)
+    assert relerr(y, np.concatenate([ref["m"].T, ref["p"].T], axis=1)) <= tol
