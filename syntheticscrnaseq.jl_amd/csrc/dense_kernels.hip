@@ -150,7 +150,7 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
     for (int b = 0; b < NB; ++b)
 #pragma unroll
         for (int h = 0; h < 2; ++h)
-            pfx[b][h] = philox_fixed((uint32_t)cell, (uint32_t)(gene(b, 2 * h) >> 1), c3n, k1);
+            pfx[b][h] = philox_fixed((uint32_t)cell, (uint32_t)(gene(b, 2 * h) >> 1), c3n, k0, k1);
 
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
@@ -253,6 +253,16 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
                 for (int v = 0; v < 4; ++v) {
                     // Padding genes (g >= G) keep p = 0, so act = 0 and rep = 1.
                     const float x = p[b][v] * invK;
+                    if constexpr (MODE != 3) {
+                        // act = 1 - rep: one fma forms 1 + x^2, and x^n = inf gives
+                        // rep = 0, act = 1 without a clamp.  Its absolute error
+                        // (2^-24) is far below the bf16 rounding of act.
+                        const float d = H2 ? fmaf(x, x, 1.0f)
+                                           : 1.0f + __builtin_amdgcn_exp2f(hill_n * __builtin_amdgcn_logf(x));
+                        rep[v] = __builtin_amdgcn_rcpf(d);
+                        act[v] = 1.0f - rep[v];
+                        continue;
+                    }
                     float xn = H2 ? x * x : __builtin_amdgcn_exp2f(hill_n * __builtin_amdgcn_logf(x));
                     xn = fminf(xn, 1e30f);
                     rep[v] = __builtin_amdgcn_rcpf(1.0f + xn);
