@@ -139,6 +139,55 @@ __device__ __forceinline__ U4 philox4x32_10_fixed(const PhiloxFixed& f, uint32_t
     return U4{c0, c1r, c2, c3};
 }
 
+// Blocks whose c0 is a lane's fixed value (the cell) and c3 fixed, with c1
+// (the step) and c2 (the gene pair) the same in every lane of the wave, as in
+// the FP32 stepper, where a wave works on the same genes of 64 cells.  Round
+// 0's lane product M0*c0 and round 1's M1*c2 (c2 then being hi(M0*c0) ^ c3 ^ k)
+// are the same for every step and pair; with the round keys they are xored
+// with, three words per lane.  The uniform products (M1*c2, then M0*c0 of
+// round 1, whose c0 is hi(M1*c2) ^ c1 ^ k) go to the scalar unit, so a block
+// costs 16 lane products.  Same block as philox4x32_10(c0, c1, c2, c3, k0, k1).
+struct PhiloxLane {
+    uint32_t b;   // lo(M0*c0) ^ k1_1          round 1: c2 = hi(M0*c0_1) ^ b
+    uint32_t x;   // hi(M1*c2_1) ^ k0_1        round 1: c0 = x ^ lo(M1*c2)
+    uint32_t y;   // lo(M1*c2_1) ^ k0_2        round 2: c0 = hi(M1*c2_2) ^ y
+};
+__device__ __forceinline__ PhiloxLane philox_lane(uint32_t c0, uint32_t c3, uint32_t k0, uint32_t k1) {
+    const uint64_t p0 = (uint64_t)kPhiloxM0 * c0;
+    const uint64_t q1 = (uint64_t)kPhiloxM1 * ((uint32_t)(p0 >> 32) ^ c3 ^ k1);
+    return PhiloxLane{(uint32_t)p0 ^ (k1 + kPhiloxW1), (uint32_t)(q1 >> 32) ^ (k0 + kPhiloxW0),
+                      (uint32_t)q1 ^ (k0 + 2u * kPhiloxW0)};
+}
+// Block (c0, c1, c2, c3) from the lane part; c1, c2 and the key wave-uniform.
+__device__ __forceinline__ U4 philox4x32_10_lane(const PhiloxLane& f, uint32_t c1u, uint32_t c2u, uint32_t k0,
+                                                 uint32_t k1) {
+    const uint64_t u1 = (uint64_t)kPhiloxM1 * c2u;                               // round 0, uniform
+    const uint64_t u2 = (uint64_t)kPhiloxM0 * ((uint32_t)(u1 >> 32) ^ c1u ^ k0);   // round 1, uniform
+    const uint32_t c0_2 = f.x ^ (uint32_t)u1;
+    const uint32_t c2_2 = (uint32_t)(u2 >> 32) ^ f.b;
+    const uint64_t p0 = (uint64_t)kPhiloxM0 * c0_2;                              // round 2
+    const uint64_t p1 = (uint64_t)kPhiloxM1 * c2_2;
+    uint32_t c0 = (uint32_t)(p1 >> 32) ^ f.y;
+    uint32_t c2 = (uint32_t)(p0 >> 32) ^ ((uint32_t)u2 ^ (k1 + 2u * kPhiloxW1));
+    uint32_t c1r = (uint32_t)p1, c3 = (uint32_t)p0;
+    k0 += 3u * kPhiloxW0;
+    k1 += 3u * kPhiloxW1;
+#pragma unroll
+    for (int r = 3; r < 10; ++r) {
+        const uint64_t q0 = (uint64_t)kPhiloxM0 * c0;
+        const uint64_t q1 = (uint64_t)kPhiloxM1 * c2;
+        const uint32_t n0 = xor3((uint32_t)(q1 >> 32), c1r, k0);
+        const uint32_t n2 = xor3((uint32_t)(q0 >> 32), c3, k1);
+        c1r = (uint32_t)q1;
+        c3 = (uint32_t)q0;
+        c0 = n0;
+        c2 = n2;
+        k0 += kPhiloxW0;
+        k1 += kPhiloxW1;
+    }
+    return U4{c0, c1r, c2, c3};
+}
+
 // 1 + (w >> 9) * 2^-23 as a float in [1, 2): the top 23 bits of w placed
 // under the exponent of 1.0 by one v_alignbit_b32 ((0x7f:w) >> 9).
 __device__ __forceinline__ float one_plus_u23(uint32_t w) {
