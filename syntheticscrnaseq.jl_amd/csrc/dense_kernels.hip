@@ -308,7 +308,10 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
         float nr2[NB][4], ncs[NB][4], nsn[NB][4];
         uint32_t nwa[NB][4], nwb[NB][4];
         auto draw = [&](const int b, const int h) {
-            const U4 u = philox4x32_10_fixed(pfx[b][h], (uint32_t)s, k0, k1);
+            // Round keys: recomputed on the scalar unit at each call in the
+            // one-block forms, kept in SGPRs by BF16X3 (whose 2-wave budget has
+            // the scalar registers; 95 -> 91 ms, while BF16 measured 61 -> 65 ms).
+            const U4 u = philox4x32_10_fixed<MODE != 2>(pfx[b][h], (uint32_t)s, k0, k1);
             normal_pair_r2(u.x, u.y, neg2ln2dt, nr2[b][2 * h], ncs[b][2 * h], nsn[b][2 * h]);
             normal_pair_r2(u.z, u.w, neg2ln2dt, nr2[b][2 * h + 1], ncs[b][2 * h + 1], nsn[b][2 * h + 1]);
             nwa[b][2 * h] = u.x;

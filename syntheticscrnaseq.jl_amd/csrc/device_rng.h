@@ -103,10 +103,12 @@ __device__ __forceinline__ PhiloxFixed philox_fixed(uint32_t c0, uint32_t c2, ui
                        (uint32_t)q1 ^ (k0 + 2u * kPhiloxW0), (uint32_t)(q2 >> 32) ^ (k1 + 2u * kPhiloxW1),
                        (uint32_t)q2 ^ (k1 + 3u * kPhiloxW1)};
 }
-// Block (c0, c1, c2, c3) from its fixed part, key wave-uniform (kept opaque
-// as in philox4x32_10_uk).
+// Block (c0, c1, c2, c3) from its fixed part, key wave-uniform.  OPAQUE keeps
+// the key opaque as in philox4x32_10_uk (round keys recomputed on the scalar
+// unit at each call); without it the compiler may keep them in SGPRs.
+template <bool OPAQUE = true>
 __device__ __forceinline__ U4 philox4x32_10_fixed(const PhiloxFixed& f, uint32_t c1, uint32_t k0, uint32_t k1) {
-    asm volatile("" : "+s"(k0), "+s"(k1));
+    if constexpr (OPAQUE) asm volatile("" : "+s"(k0), "+s"(k1));
     const uint64_t r1 = (uint64_t)kPhiloxM0 * (f.a ^ c1);        // round 1
     const uint64_t r2 = (uint64_t)kPhiloxM1 * ((uint32_t)(r1 >> 32) ^ f.b);   // round 2
     uint32_t c0 = (uint32_t)(r2 >> 32) ^ f.c, c1r = (uint32_t)r2;
