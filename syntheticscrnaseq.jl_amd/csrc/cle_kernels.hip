@@ -338,9 +338,11 @@ cle_stepper_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ 
     if (pop) V = vol_pool[off_vol + cell];   // written by the population init kernel or a previous phase
     // The lane's step-invariant part of its normal blocks (cell, s, pair): three
     // VGPRs that save two lane products per block (G=20 benchmark, 4 genes per
-    // lane: 20.0 -> 19.3 ms).  Wider tiles, whose registers are already
-    // short, draw the blocks whole.
-    constexpr bool kLaneRng = CH <= 4;
+    // lane: 20.0 -> 19.3 ms).  Wider tiles, whose registers are already short,
+    // and population launches, whose short phases pay for the extra spills at
+    // every start (4096-network batch at T=20: 1.97 -> 2.13 s), draw the
+    // blocks whole.
+    constexpr bool kLaneRng = CH <= 4 && !POP;
     PhiloxLane plane{0u, 0u, 0u};
     if constexpr (kLaneRng)
         plane = philox_lane((uint32_t)cell, (B[H_NETID] << 4) | kNormal, B[H_SEED_LO], B[H_SEED_HI]);
