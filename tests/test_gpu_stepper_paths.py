@@ -81,6 +81,23 @@ def test_overflow_edges_and_wide_gates_every_tile(ch):
     assert err <= 1e-5
 
 
+@pytest.mark.parametrize("ch", [8, 16])
+def test_many_streamed_edges_wide_tiles(ch):
+    """Wide tiles stream in-edges eight per round with a one-edge tail: 9 to 27
+    in-edges per gene cover one to three rounds and every tail length."""
+    rng = np.random.default_rng(60 + ch)
+    net = wide_network(40, rng, per_gene=(9, 27))
+    ctx2 = sc.Context(0)
+    try:
+        ctx2.set_tile(1, 64, ch)
+        print(f"many edges G=40 tile {ch}", flush=True)
+        err = run_both(net, 70, 0.01, 1.0, 3 + ch, precision="fp32_valu", ctx=ctx2)
+    finally:
+        ctx2.close()
+    print(f"max rel err {err:.3e}")
+    assert err <= 1e-5
+
+
 @pytest.mark.parametrize("three_source", [True, False])
 def test_dense_gate_layouts(ctx, three_source):
     rng = np.random.default_rng(7 if three_source else 8)
