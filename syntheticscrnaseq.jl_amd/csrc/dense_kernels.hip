@@ -105,11 +105,20 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
     const int cl = l & 15;                 // the lane's cell: row cl of the tile
     const int qd = l >> 4;                 // its gene quad: genes 4*qd .. 4*qd+3 of each block
 
-    // LDS: act tiles [plane][buffer][16 rows][Gp + 8] bf16 (FP32: [buffer][16 rows][Gp + 4] f32, rows
-    // 16 bytes apart in bank order so that the 16-byte reads of 16 lanes hit distinct banks), then FP32
-    // gate Hill values [buffer][2 S][16 cells].
+    // LDS: act tiles [plane][buffer][16 rows][Gp + 16] bf16 (FP32: [buffer][16 rows][Gp + 4] f32), then
+    // FP32 gate Hill values [buffer][2 S][16 cells].  In the BF16 tile, rows whose cell has bit 2 set
+    // hold their 16-byte chunks pairwise swapped (column c at c ^ 8).  Measured on gfx950 with the
+    // bank-conflict counter (SQ_LDS_BANK_CONFLICT), for these exact lane patterns: rows 8 dwords
+    // (mod 16) apart make the B-fragment ds_read_b128 conflict-free (rows 4 dwords apart, Gp + 8,
+    // conflict 2-way on every read, with or without a swap), and the swap brings the act write
+    // (ds_write_b64, served 16 consecutive lanes at a time over 32 banks) from 4-way to 2-way, the
+    // least for 16 lanes writing the same 8-byte half of their chunks.
     constexpr bool F32 = MODE == 3;
-    const int rowstride = F32 ? Gp + 4 : Gp + 8;
+    const int rowstride = F32 ? Gp + 4 : Gp + 16;
+    // BF16 tiles: the row's chunk swap, in bf16 columns.  BF16X3 leaves it out: at its 256-VGPR
+    // budget the swap's address register spills (36 B per lane), and its two planes' writes cost
+    // less than that (4-way on 2 writes per step against 16 conflict-free reads).
+    const int swz = MODE == 2 ? 0 : (cl & 4) << 1;
     const int tile_elems = 16 * rowstride;
     const int KT = Gp / 4;                 // FP32 mode: MFMAs per gene block and step
     __bf16* act_t = (__bf16*)dlds;
@@ -281,12 +290,12 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
                 bf16x4 hi;
 #pragma unroll
                 for (int v = 0; v < 4; ++v) hi[v] = (__bf16)act[v];
-                *(bf16x4*)(T0 + cl * rowstride + col) = hi;
+                *(bf16x4*)(T0 + cl * rowstride + (col ^ swz)) = hi;
                 if (MODE == 2) {
                     bf16x4 lo;
 #pragma unroll
                     for (int v = 0; v < 4; ++v) lo[v] = (__bf16)(act[v] - (float)hi[v]);
-                    *(bf16x4*)(T1 + cl * rowstride + col) = lo;
+                    *(bf16x4*)(T1 + cl * rowstride + (col ^ swz)) = lo;
                 }
             }
 #pragma unroll
@@ -364,8 +373,8 @@ cle_dense_kernel(const uint32_t* __restrict__ blobs, const Tile* __restrict__ ti
         if constexpr (!F32) {
             // With the chunk count a compile-time constant, the act reads of all
             // chunks issue ahead of the MFMAs.
-            const __bf16* arow0 = T0 + cl * rowstride + 8 * qd;
-            const __bf16* arow1 = T1 + cl * rowstride + 8 * qd;
+            const __bf16* arow0 = T0 + cl * rowstride + ((8 * qd) ^ swz);   // + 32 kc keeps bit 3
+            const __bf16* arow1 = T1 + cl * rowstride + ((8 * qd) ^ swz);
             auto contract = [&](auto kcn) {
                 constexpr int KCN = decltype(kcn)::value;
 #pragma unroll
@@ -541,7 +550,7 @@ int dense_nb(int mode) { return mode == 2 ? 2 : 1; }
 
 size_t dense_lds_bytes(int Gp, int S, int mode) {
     const size_t tiles = mode == 3 ? (size_t)2 * 16 * (Gp + 4) * sizeof(float)
-                                   : (size_t)(mode == 2 ? 4 : 2) * 16 * (Gp + 8) * 2;
+                                   : (size_t)(mode == 2 ? 4 : 2) * 16 * (Gp + 16) * 2;
     return tiles + (size_t)2 * 2 * S * 16 * sizeof(float);
 }
 
