@@ -389,6 +389,64 @@ def run_g256(a, sc) -> dict:
     return out
 
 
+def run_wide(a, sc) -> dict:
+    """Networks above the fused steppers' 256 genes: the wide path (one library
+    GEMM and one fused elementwise kernel per step, FP32 / BF16 / BF16x3 products)
+    against the reference-structured FP32 path, on the G=256 workload's network
+    construction at --genes genes (65,536 cells, 2000 steps, dt=0.005)."""
+    from scrna_amd import samplers
+
+    G = a.genes
+    net = samplers.dense_network(G)
+    kin = samplers.WORKLOAD_KINETICS
+    cells = a.cells or 65536
+    dt, steps = 0.005, 2000
+    T = steps * dt
+    cgs = float(G) * cells * steps
+    out = {"workload": f"G={G} additive network, 25% dense A, 8 AND + 8 OR, {cells} cells, {steps} steps dt=0.005",
+           "cells": cells, "steps": steps, "results": {}}
+    subsets = {}
+    n_sub = min(a.subset, 32)            # the FP64 replay costs G^2/4 edge updates per cell and step
+    precs = []
+    for p in a.precisions.split(","):
+        p = "fp32" if p == "fp32_mfma" else p
+        if p in ("fp32_unfused", "fp32", "bf16", "bf16x3") and p not in precs:
+            precs.append(p)
+    for prec in precs:
+        plan = sc.Plan([net], sc.CLE(dt), kin, cell_num=cells, T=T, readout="both", seed=777, precision=prec)
+        n_runs = min(a.steps, 3)
+        el, kms = timed_runs(plan, n_runs, min(a.warmup, 1))
+        k_ms = float(np.mean(kms))
+        r = {"cgs_per_s": cgs * n_runs / el, "kernel_ms": k_ms, "runs": n_runs, "ms_per_run": el * 1e3 / n_runs,
+             "gemm_tflops": 2.0 * G * cgs * (3 if prec == "bf16x3" else (2 if prec == "fp32_unfused" else 1))
+             / (k_ms * 1e-3) / 1e12}
+        if n_sub > 0:
+            subsets[prec] = plan.download(0).y[:n_sub]
+        out["results"][prec] = r
+        log(f"{prec}: {r['cgs_per_s']:.3e} c*g*s/s, {k_ms:.1f} ms per run")
+        plan.close()
+    if subsets and not a.no_cpu:
+        from oracle import oracle as orc
+        ref = orc.cle(net, kin, dt=dt, n_steps=steps, cell_num=cells, cell_lo=0, cell_hi=n_sub, seed=777,
+                      threads=max(1, min(16, os.cpu_count() or 1)))
+        for prec, y in subsets.items():
+            errs = {}
+            for k, want in ((0, ref["m"]), (1, ref["p"])):
+                got = y[:, k * G:(k + 1) * G].T
+                me = np.abs(got.mean(1) - want.mean(1)) / np.maximum(np.abs(want.mean(1)), 1e-3)
+                ve = np.abs(got.var(1) - want.var(1)) / np.maximum(want.var(1), 1e-3)
+                errs[("mrna", "protein")[k]] = {"max_mean_rel_err": float(me.max()), "max_var_rel_err": float(ve.max())}
+            out["results"][prec]["vs_fp64_replay"] = errs
+        out["replay_cells"] = n_sub
+    res = out["results"]
+    for prec in ("fp32", "bf16", "bf16x3"):
+        if prec in res and "fp32_unfused" in res:
+            res[prec]["speedup_vs_fp32_reference_structured"] = res[prec]["cgs_per_s"] / res["fp32_unfused"]["cgs_per_s"]
+        if prec in res and "fp32" in res:
+            res[prec]["speedup_vs_fp32_wide"] = res[prec]["cgs_per_s"] / res["fp32"]["cgs_per_s"]
+    return out
+
+
 def run_headline(a, sc) -> dict:
     """The metric line: value = BF16-MFMA throughput at G=256 with the precision
     comparison, the G=20 workload and the serving workload as fields."""
@@ -674,7 +732,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["headline", "g20", "g256", "batch", "serve", "copy", "tauleap", "sbi"],
+    ap.add_argument("--workload", choices=["headline", "g20", "g256", "batch", "serve", "copy", "tauleap", "sbi",
+                                           "wide"],
                     default="headline")
     ap.add_argument("--cells", type=int, default=None, help="override cell count (testing only)")
     ap.add_argument("--tile", type=int, default=0, help="genes per lane for the FP32 stepper (2, 4, 8, 16)")
@@ -691,6 +750,7 @@ def main() -> int:
     ap.add_argument("--qps", type=float, default=4.0, help="serve: arrival rate")
     ap.add_argument("--requests", type=int, default=60, help="serve: number of requests")
     ap.add_argument("--bytes", type=int, default=1 << 30, help="copy: bytes per copy")
+    ap.add_argument("--genes", type=int, default=1024, help="wide: genes of the network (above 256)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the library has no multi-GPU path "
@@ -707,7 +767,7 @@ def main() -> int:
         sc.default_context().set_tile(1, min(256, 16 * a.tile), a.tile)
     a.headline_precision = None
     runner = {"headline": run_headline, "g20": run_g20, "g256": run_g256, "batch": run_batch, "serve": run_serve,
-              "tauleap": run_tauleap, "sbi": run_sbi}[a.workload]
+              "tauleap": run_tauleap, "sbi": run_sbi, "wide": run_wide}[a.workload]
     print(json.dumps(runner(a, sc)))
     return 0
 

@@ -52,7 +52,10 @@ typedef enum scrna_precision {
     SCRNA_PREC_FP32 = 0,        /* FP32 state and regulation; the library picks the form: the dense matrix-core
                                    contraction (SCRNA_PREC_FP32_MFMA) for a CLE network without population
                                    dynamics with G >= 128 and at least 16 additive in-edges per gene on
-                                   average, the sparse VALU stepper (SCRNA_PREC_FP32_VALU) otherwise */
+                                   average, or with G > 256; the sparse VALU stepper (SCRNA_PREC_FP32_VALU)
+                                   otherwise.  Above 256 genes the BF16, BF16X3 and FP32_MFMA forms run as
+                                   one library GEMM and one fused elementwise kernel per step (the wide
+                                   path), without population dynamics */
     SCRNA_PREC_BF16 = 1,        /* dense regulation on BF16 MFMA, FP32 accumulate */
     SCRNA_PREC_BF16X3 = 2,      /* dense regulation as hi*hi + hi*lo + lo*hi BF16 MFMA */
     SCRNA_PREC_FP32_UNFUSED = 3,/* the reference GPU structure (ext:87-108): one kernel per

@@ -21,7 +21,7 @@ OUT = os.path.join(HERE, "libscrna.so")
 OBJDIR = os.path.join(HERE, "_build")
 
 SOURCES = ["cle_kernels.hip", "runtime.cpp", "population.cpp", "server.cpp", "dense_kernels.hip",
-           "autotune.cpp", "unfused.hip", "tauleap_kernels.hip", "prior.cpp",
+           "autotune.cpp", "unfused.hip", "wide.hip", "tauleap_kernels.hip", "prior.cpp",
            "mtx_writer.cpp", "noalloc.cpp"]
 
 COMMON = [

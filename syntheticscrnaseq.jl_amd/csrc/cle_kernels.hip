@@ -27,6 +27,7 @@
 #include "device_binomial.h"
 #include "device_rng.h"
 #include "cle_kernels.h"
+#include "hill.h"
 #include "segment_schedule.h"
 
 namespace scrna {
@@ -34,25 +35,6 @@ namespace scrna {
 __device__ __forceinline__ float as_f(uint32_t w) { return __uint_as_float(w); }
 __device__ __forceinline__ uint64_t u64_at(const uint32_t* B, int i) {
     return (uint64_t)B[i] | ((uint64_t)B[i + 1] << 32);
-}
-
-// Hill pair for one protein value: act = x^n/(1+x^n), rep = 1/(1+x^n) with
-// x = p/K (network.jl:112-125; same quantities as cle.jl:79-82 divided
-// through by K^n, which keeps x^n finite for large p).
-// HILL2 selects n = 2 at compile time; callers branch on the (uniform) flag
-// outside their gene loops so that the n = 2 path never evaluates log/exp.
-template <bool HILL2>
-__device__ __forceinline__ void hill_pair(float p, float invK, float hill_n, float& act, float& rep) {
-    const float x = p * invK;
-    float xn;
-    if (HILL2) {
-        xn = x * x;
-    } else {
-        xn = __builtin_amdgcn_exp2f(hill_n * __builtin_amdgcn_logf(x));
-    }
-    xn = fminf(xn, 1e30f);
-    rep = __builtin_amdgcn_rcpf(1.0f + xn);
-    act = xn * rep;
 }
 
 // Hill values of a wave's source genes (bit k of srcm: gene g0+k) -> the LDS

@@ -124,19 +124,16 @@ int validate_inputs(const scrna_network* net, const scrna_kinetics* kin, const s
     const int G = net->G;
     if (G <= 0) return set_error(SCRNA_ERR_INVALID, "G must be positive (got %d)", G);
     if (G > kMaxGenes) {
-        // Past the fused steppers' limit the CLE runs on the reference-structured FP32 path
-        // (rocBLAS products for any G, like ext:93-94), which has no telegraph promoters,
-        // population dynamics or injected noise.
-        bool tele = false;
-        if (net->k_on)
-            for (int g = 0; g < G; ++g) tele |= std::isfinite(net->k_on[g]) || std::isfinite(net->k_off[g]);
-        if (G > kMaxGenesUnfused || run->algorithm != SCRNA_ALG_CLE ||
-            (run->precision != SCRNA_PREC_FP32 && run->precision != SCRNA_PREC_FP32_UNFUSED) || tele ||
-            (pop && pop->enabled) || run->injected_noise)
+        // Past the fused steppers' limit the CLE runs on the wide path (one library GEMM
+        // and one fused elementwise kernel per step: FP32, BF16 or BF16x3 products, like
+        // ext:93-94 for any G) or, when asked for, on the reference-structured FP32 path.
+        // Neither holds population dynamics, the tau-leap or the sparse VALU stepper.
+        if (G > kMaxGenesUnfused || run->algorithm != SCRNA_ALG_CLE || run->precision == SCRNA_PREC_FP32_VALU ||
+            (pop && pop->enabled))
             return set_error(SCRNA_ERR_UNSUPPORTED,
-                             "G=%d exceeds the fused steppers' maximum %d: it runs as the FP32 CLE on the "
-                             "reference-structured path (precision fp32 or fp32_unfused, G <= %d) without telegraph "
-                             "promoters, population dynamics or injected noise",
+                             "G=%d exceeds the fused steppers' maximum %d: it runs as a CLE without population "
+                             "dynamics on the wide path (precision fp32, fp32_mfma, bf16 or bf16x3) or the "
+                             "reference-structured path (fp32_unfused), G <= %d",
                              G, kMaxGenes, kMaxGenesUnfused);
     }
     if (!net->basals || !net->A) return set_error(SCRNA_ERR_INVALID, "basals and A are required");
@@ -277,6 +274,7 @@ int compile_network(const scrna_network* net, const scrna_kinetics* kin, const s
     const bool dense = run->precision == SCRNA_PREC_BF16 || run->precision == SCRNA_PREC_BF16X3 ||
                        run->precision == SCRNA_PREC_FP32_MFMA;
     out->dense = dense;
+    out->wide = dense && G > kMaxGenes;   // wide path: no tiles, no packed MFMA operands
     out->mode = run->precision == SCRNA_PREC_FP32_MFMA ? 3 : run->precision;
     out->Gp = (G + 31) / 32 * 32;
     std::vector<double> Wd, bias;
@@ -568,7 +566,9 @@ int compile_network(const scrna_network* net, const scrna_kinetics* kin, const s
     b[H_OFF_BIAS] = (uint32_t)b.size();
     for (int g = 0; g < G; ++g) b.push_back(f2w(dense ? (float)bias[g] : 0.0f));
     align16();
-    if (dense && out->mode == 3) {
+    if (out->wide) {
+        // the wide path uploads W itself, column-major (wide.hip)
+    } else if (dense && out->mode == 3) {
         // FP32 W^T as v_mfma_f32_16x16x4_f32 B operands: [gene block][t][lane] f32, lane l
         // of block blk holding W[16 blk + (l&15)][(l>>4) Gp/4 + t] (the kernel's K order).
         const int Gp = out->Gp, NBLK = Gp / 16, KT = Gp / 4;
@@ -630,6 +630,7 @@ void plan_release(scrna_plan* p) {
     if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
     p->ev_fork = nullptr;
     for (auto& u : p->unfused) unfused_release(u);
+    for (auto& w : p->wide) wide_release(w);
     if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     p->graph_exec = nullptr;
@@ -644,7 +645,7 @@ void plan_release(scrna_plan* p) {
 // cost grows with the in-degree rather than with G.
 static int effective_precision(const scrna_network& net, const scrna_run& run, bool population) {
     if (run.precision == SCRNA_PREC_FP32_VALU) return SCRNA_PREC_FP32;
-    if (run.precision == SCRNA_PREC_FP32 && net.G > kMaxGenes) return SCRNA_PREC_FP32_UNFUSED;
+    if (run.precision == SCRNA_PREC_FP32 && net.G > kMaxGenes) return SCRNA_PREC_FP32_MFMA;   // wide path
     if (run.precision != SCRNA_PREC_FP32 || run.algorithm != SCRNA_ALG_CLE || population || net.G < 128)
         return run.precision;
     int64_t nnz = 0;
@@ -690,11 +691,11 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
         // Arena plans run on one stream with no events, and hold no
         // population buffers or library handles of their own.
         for (int i = 0; i < n; ++i)
-            if ((pops && pops[i].enabled) ||
+            if ((pops && pops[i].enabled) || nets[i].G > kMaxGenes ||
                 effective_precision(nets[i], runs[i], false) == SCRNA_PREC_FP32_UNFUSED)
                 return set_error(SCRNA_ERR_UNSUPPORTED,
-                                 "population runs and the reference-structured FP32 path (also every network with "
-                                 "G > %d) allocate their own buffers", kMaxGenes);
+                                 "population runs, the reference-structured FP32 path and the wide path (every "
+                                 "network with G > %d) allocate their own buffers", kMaxGenes);
     }
     P->nets.resize(n);
     std::vector<uint32_t> blobs;
@@ -780,6 +781,19 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
                     u.host_A[(size_t)G * G + a + (size_t)b2 * G] = (float)std::max(-aeff, 0.0);   // A_neg
                 }
             P->unfused.push_back(std::move(u));
+            continue;
+        }
+        if (cn.wide) {
+            // Wide path: W = A_eff (network.jl:212) column-major, the element (g, k) at g + k G.
+            WideNet w;
+            w.net = i;
+            w.mode = cn.mode;
+            const int G = cn.G;
+            w.host_W.assign((size_t)G * G, 0.0f);
+            for (int a = 0; a < G; ++a)
+                for (int b2 = 0; b2 < G; ++b2)
+                    if (a != b2) w.host_W[a + (size_t)b2 * G] = (float)(nets[i].A[a + (size_t)b2 * G] * nets[i].basals[a]);
+            P->wide.push_back(std::move(w));
             continue;
         }
         const bool tele = (cn.blob[H_FLAGS] & F_TELE) != 0, inj = (cn.blob[H_FLAGS] & F_INJECT) != 0;
@@ -970,6 +984,12 @@ int plan_build(scrna_ctx* ctx, int32_t n, const scrna_network* nets, const scrna
     P->n_cjobs = (int)cjobs.size();
     for (auto& u : P->unfused) {
         if ((rc = unfused_prepare(P.get(), u))) {
+            plan_release(P.get());
+            return rc;
+        }
+    }
+    for (auto& w : P->wide) {
+        if ((rc = wide_prepare(P.get(), w))) {
             plan_release(P.get());
             return rc;
         }
@@ -1175,6 +1195,8 @@ int plan_enqueue(scrna_plan* P, hipStream_t st, bool events) {
         if (rc) return rc;
         for (auto& u : P->unfused)
             if ((rc = unfused_launch(P, u, st))) return rc;
+        for (auto& w : P->wide)
+            if ((rc = wide_launch(P, w, st))) return rc;
     }
     if (events) HIP_TRY(hipEventRecord(P->ev_stop, st));
     HIP_TRY(launch_moments((int)P->mjobs_snap.size(), P->max_mom_G, P->d_mjobs_snap, P->d_snap, P->d_mom, st));
@@ -1215,8 +1237,9 @@ int plan_check_schedule(scrna_plan* P) {
 // replay it with a single hipGraphLaunch.  No memset nodes are captured.
 int plan_make_graph(scrna_plan* P) {
     if (P->graph_exec) return SCRNA_OK;
-    if (!P->unfused.empty())
-        return set_error(SCRNA_ERR_UNSUPPORTED, "the reference-structured FP32 path is not captured into graphs");
+    if (!P->unfused.empty() || !P->wide.empty())
+        return set_error(SCRNA_ERR_UNSUPPORTED,
+                         "the reference-structured and wide paths (library GEMMs) are not captured into graphs");
     HIP_TRY(hipSetDevice(P->ctx->device));
     hipStream_t cs = nullptr;
     HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));

@@ -35,7 +35,7 @@ struct scrna_ctx {
 namespace scrna {
 
 constexpr int kMaxGenes = 256;            // fused steppers (registers / LDS hold a network's genes)
-constexpr int kMaxGenesUnfused = 16384;    // reference-structured FP32 path (rocBLAS products, any G)
+constexpr int kMaxGenesUnfused = 16384;    // wide and reference-structured paths (rocBLAS products)
 constexpr uint64_t kSnapPoolBit = 1ull << 63;
 
 struct CompiledNet {
@@ -56,6 +56,19 @@ struct CompiledNet {
     int mode = 0, Gp = 0;
     std::vector<uint16_t> wpack;     // bf16 bits
     size_t off_w = 0;                // in 8-element (16-byte) units
+    bool wide = false;               // G > kMaxGenes: library GEMM + fused elementwise kernel (wide.hip)
+};
+
+// A network on the wide path (wide.hip): G above the fused steppers' limit,
+// one library GEMM and one fused elementwise kernel per step.
+struct WideNet {
+    int net = 0;
+    int mode = 3;                     // 1 BF16, 2 BF16X3, 3 FP32
+    std::vector<float> host_W;        // W = A_pos - A_neg, column-major G x G (until uploaded)
+    float* reg = nullptr;
+    float* gh = nullptr;
+    void *act = nullptr, *act_lo = nullptr, *W = nullptr, *W_lo = nullptr;
+    void* blas = nullptr;             // rocblas_handle
 };
 
 // A network on the reference-structured FP32 path (unfused.hip).
@@ -99,6 +112,7 @@ struct scrna_plan {
     std::vector<scrna::CompiledNet> nets;
     std::vector<scrna::LaunchGroup> groups;
     std::vector<scrna::UnfusedNet> unfused;
+    std::vector<scrna::WideNet> wide;
     std::vector<std::pair<void*, size_t>> allocs;
     uint64_t bytes = 0;
     double cell_gene_steps = 0.0;
@@ -175,6 +189,9 @@ int plan_make_graph(scrna_plan* P);
 int unfused_prepare(scrna_plan* P, UnfusedNet& u);
 int unfused_launch(scrna_plan* P, UnfusedNet& u, hipStream_t st);
 void unfused_release(UnfusedNet& u);
+int wide_prepare(scrna_plan* P, WideNet& w);
+int wide_launch(scrna_plan* P, WideNet& w, hipStream_t st);
+void wide_release(WideNet& w);
 int ctx_tile_for(scrna_ctx* ctx, int G);
 int population_prepare(scrna_plan* P, size_t state_elems, size_t prom_elems, size_t vol_elems);
 int plan_alloc(scrna_plan* p, void** ptr, size_t bytes);

@@ -92,8 +92,14 @@ def gpu_g20_workload(seed: int = 1):
 def dense_g256_network(seed: int = 2) -> GeneNetwork:
     """G=256 additive network, 25% nonzeros at seeded positions, +-U(0.5,3),
     plus 8 AND and 8 OR edges, constitutive."""
+    return dense_network(256, seed)
+
+
+def dense_network(G: int, seed: int = 2) -> GeneNetwork:
+    """Additive network of G genes, 25% nonzeros at seeded positions, +-U(0.5,3),
+    plus 8 AND and 8 OR edges, constitutive (the G=256 precision workload's
+    construction at any size)."""
     rng = np.random.default_rng(seed)
-    G = 256
     basals = rng.uniform(0.5, 3.0, size=G)
     mask = rng.random((G, G)) < 0.25
     np.fill_diagonal(mask, False)
