@@ -275,7 +275,7 @@ def run_g20(a, sc) -> dict:
     k_ms = float(np.mean(kernel_ms))
     census = valu_census(net)
     achieved = census["fp32_equiv_flops"] * cgs_per_run / (k_ms * 1e-3) / 1e12
-    traffic = measured_traffic("cle_stepper_g20_traffic.json", ("cle_kernels.hip", "device_rng.h", "blob.h"),
+    traffic = measured_traffic("cle_stepper_g20_traffic.json", ("cle_kernels.hip", "hill.h", "device_rng.h", "blob.h"),
                                wl["cells"] == 100_000)
     out = {
         "metric": "cell*gene*steps/s (GPU CLE, FP32 state)",
