@@ -420,7 +420,7 @@ def run_wide(a, sc) -> dict:
         r = {"cgs_per_s": cgs * n_runs / el, "kernel_ms": k_ms, "runs": n_runs, "ms_per_run": el * 1e3 / n_runs,
              "gemm_tflops": 2.0 * G * cgs * (3 if prec == "bf16x3" else (2 if prec == "fp32_unfused" else 1))
              / (k_ms * 1e-3) / 1e12}
-        if n_sub > 0:
+        if n_sub > 0 and not a.no_cpu:
             subsets[prec] = plan.download(0).y[:n_sub]
         out["results"][prec] = r
         log(f"{prec}: {r['cgs_per_s']:.3e} c*g*s/s, {k_ms:.1f} ms per run")
@@ -471,6 +471,20 @@ def run_headline(a, sc) -> dict:
     log("tau-leap workload")
     tl_args = argparse.Namespace(**{**vars(a), "steps": min(a.steps, 3), "warmup": 1, "no_cpu": True})
     tl = run_tauleap(tl_args, sc)
+    wide = None
+    if not a.no_wide:
+        log("wide workload (G=1024)")
+        w_args = argparse.Namespace(**{**vars(a), "steps": 1, "warmup": 1, "no_cpu": True, "genes": 1024,
+                                       "precisions": "fp32_unfused,fp32,bf16"})
+        try:
+            wr = run_wide(w_args, sc)
+            wide = {"workload": wr["workload"],
+                    **{p: {k: r[k] for k in ("cgs_per_s", "kernel_ms") if k in r} for p, r in wr["results"].items()},
+                    "bf16_vs_fp32_reference_structured":
+                        wr["results"]["bf16"].get("speedup_vs_fp32_reference_structured"),
+                    "bf16_vs_fp32_wide": wr["results"]["bf16"].get("speedup_vs_fp32_wide")}
+        except Exception as e:      # the headline stands without this field
+            wide = {"error": f"{type(e).__name__}: {e}"}
     out = {
         "metric": "cell*gene*steps/s, BF16-MFMA vs FP32 CLE at <=1% moment err; p99 latency",
         "value": bf["cgs_per_s"],
@@ -501,6 +515,7 @@ def run_headline(a, sc) -> dict:
         "g20": {k: g20[k] for k in ("value", "ms_per_step", "config", "roofline")},
         "serving": srv,
         "tauleap": {k: tl.get(k) for k in ("workload", "cgs_per_s", "kernel_ms", "roofline")},
+        "wide_g1024": wide,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s (FP32-FMA-equivalent VALU issue)", "frac": achieved / FP32_PEAK_TFLOPS,
                      "traffic": traffic, "kernel": "cle_dense_kernel (BF16 MFMA)", "kernel_ms": bf["kernel_ms"],
@@ -751,6 +766,7 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=60, help="serve: number of requests")
     ap.add_argument("--bytes", type=int, default=1 << 30, help="copy: bytes per copy")
     ap.add_argument("--genes", type=int, default=1024, help="wide: genes of the network (above 256)")
+    ap.add_argument("--no-wide", action="store_true", help="headline: skip the G=1024 wide-path field")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the library has no multi-GPU path "
