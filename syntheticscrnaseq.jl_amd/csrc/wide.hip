@@ -71,7 +71,8 @@ __global__ void __launch_bounds__(256) wide_step_kernel(WideArgs a, int s) {
     const int C = (int)B[H_CELLS];
     const int S = (int)B[H_S];
     const int lane = threadIdx.x & 63;
-    const int c0 = ((int)blockIdx.x * 64 + lane) * V;        // first cell of the lane; C % V == 0
+    const int64_t c0l = ((int64_t)blockIdx.x * 64 + lane) * V;   // first cell of the lane; C % V == 0
+    const int c0 = (int)(c0l < C ? c0l : C);
     const bool live = c0 < C;
     const int cb = live ? c0 : 0;                            // idle lanes shadow cells 0.. and store nothing
     const uint32_t flags = B[H_FLAGS];
@@ -364,7 +365,7 @@ int wide_launch(scrna_plan* P, WideNet& w, hipStream_t st) {
                   : w.mode == 2 ? wide_fn<2>(tele, inj, vec4)
                                 : wide_fn<3>(tele, inj, vec4);
     const int per_wave = vec4 ? 256 : 64;    // cells of one wave
-    const dim3 grid((unsigned)((C + per_wave - 1) / per_wave), (unsigned)(((G + 1) / 2 + 3) / 4)), block(256);
+    const dim3 grid((unsigned)(((int64_t)C + per_wave - 1) / per_wave), (unsigned)(((G + 1) / 2 + 3) / 4)), block(256);
     if (grid.y > 65535u) return set_error(SCRNA_ERR_INVALID, "wide path: G=%d needs more than 65535 block rows", G);
     WideArgs a{P->d_blobs + cn.blob_off, P->d_state, P->d_snap, P->d_prom, P->d_inj, w.reg, w.act,
                (__bf16*)w.act_lo, w.gh};
