@@ -80,3 +80,27 @@ def test_wide_path_refusals(ctx):
         sc.simulate(net, sc.CLE(0.01), KIN, population=pop, cell_num=64, T=0.1, ctx=ctx)
     with pytest.raises(sc.UnsupportedError, match="wide path"):
         sc.simulate(net, sc.CLE(0.01), KIN, cell_num=64, T=0.1, precision="fp32_valu", ctx=ctx)
+
+
+def test_wide_and_fused_networks_in_one_plan(ctx):
+    """A plan holding two wide networks (G=300, 270) beside a fused one (G=20):
+    each network's offsets into the shared pools (state, promoters, snapshots,
+    injected noise) are honoured, and every network matches the oracle on its
+    own injected noise.  The first network's 301 x 37 cell*genes put the
+    second (40 cells, a multiple of 4) at a promoter-pool offset that is not a
+    multiple of 4, so it takes the one-cell-per-lane form."""
+    nets = [network(301, 40, telegraph=True), network(270, 41), sc.samplers.gpu_g20_workload()["net"]]
+    cells, dt, T, seed = [37, 40, 64], 0.01, 0.3, 8
+    steps = sc.n_steps(T, dt)
+    rng = np.random.default_rng(seed)
+    xis = [rng.standard_normal((steps, n.G, 2, c)).astype(np.float32) for n, c in zip(nets, cells)]
+    print(f"wide + fused plan: G={[n.G for n in nets]} cells={cells}", flush=True)
+    out = sc.simulate_gpu_batch(nets, sc.CLE(dt), KIN, cell_num=cells, T=T, readout="both", seed=seed,
+                                injected_noise=xis, snapshot_every=10, full=True, ctx=ctx)
+    for i, (n, c) in enumerate(zip(nets, cells)):
+        ref = orc.cle(n, KIN, dt=dt, n_steps=steps, cell_num=c, injected=xis[i], seed=seed, net_id=i,
+                      snapshot_every=10, threads=4)
+        err = relerr(out[i].y, np.concatenate([ref["m"].T, ref["p"].T], axis=1))
+        print(f"network {i}: max rel err {err:.3e}")
+        assert err <= 1e-5
+        np.testing.assert_array_equal(out[i].snapshots[-1, 1].T.astype(np.float64), out[i].y[:, n.G:])
