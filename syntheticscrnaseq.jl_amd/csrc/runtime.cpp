@@ -278,8 +278,9 @@ int compile_network(const scrna_network* net, const scrna_kinetics* kin, const s
     out->mode = run->precision == SCRNA_PREC_FP32_MFMA ? 3 : run->precision;
     out->Gp = (G + 31) / 32 * 32;
     std::vector<double> Wd, bias;
+    const bool packed_w = dense && G <= kMaxGenes;   // the wide path uploads its own W (wide.hip)
     if (dense) {
-        Wd.assign((size_t)G * G, 0.0);
+        if (packed_w) Wd.assign((size_t)G * G, 0.0);
         bias.assign(G, 0.0);
         out->NW = out->Gp / (16 * dense_nb(out->mode));
     }
@@ -295,7 +296,7 @@ int compile_network(const scrna_network* net, const scrna_kinetics* kin, const s
             const double aeff = net->A[i + (size_t)j * G] * net->basals[i];
             if (aeff == 0.0) continue;
             if (dense) {
-                Wd[(size_t)i * G + j] = aeff;
+                if (packed_w) Wd[(size_t)i * G + j] = aeff;
                 if (aeff < 0.0) bias[i] += -aeff;
             } else {
                 in_edges[i].push_back({j, aeff});
