@@ -104,3 +104,23 @@ def test_wide_and_fused_networks_in_one_plan(ctx):
         print(f"network {i}: max rel err {err:.3e}")
         assert err <= 1e-5
         np.testing.assert_array_equal(out[i].snapshots[-1, 1].T.astype(np.float64), out[i].y[:, n.G:])
+
+
+def test_wide_path_large_network(ctx):
+    """G=4096 (512 rows of elementwise blocks, a 64 MB FP32 W): FP32 and BF16
+    against the oracle's replay of four cells on the free Philox stream."""
+    G, cells, dt, T, seed = 4096, 1024, 0.01, 0.1, 3
+    rng = np.random.default_rng(77)
+    A = np.where(rng.random((G, G)) < 0.01, rng.uniform(0.5, 3.0, (G, G)), 0.0)
+    A *= np.where(rng.random((G, G)) < 0.5, 1.0, -1.0)
+    np.fill_diagonal(A, 0.0)
+    net = sc.GeneNetwork(G, rng.uniform(0.5, 3.0, G), A, [], [])
+    ref = orc.cle(net, KIN, dt=dt, T=T, cell_num=cells, cell_lo=0, cell_hi=4, seed=seed, threads=4)
+    want = np.concatenate([ref["m"].T, ref["p"].T], axis=1)
+    for precision, bar in (("fp32", 2e-4), ("bf16", 2e-2)):
+        print(f"wide G={G} cells={cells} {precision}", flush=True)
+        y = sc.simulate(net, sc.CLE(dt), KIN, cell_num=cells, T=T, readout="both", seed=seed, precision=precision,
+                        ctx=ctx)
+        err = relerr(y[:4], want)
+        print(f"max rel err {err:.3e}")
+        assert np.isfinite(y).all() and err <= bar, precision
